@@ -1,0 +1,411 @@
+"""Python binding of the MI355X LBM solver's C ABI (include/aeros_lbm.h).
+
+The solver itself is C++ host code plus hand-written HIP kernels built into
+``aeros-engine_amd/build/libaeros_lbm.so``; this module only declares the ABI
+structs and calls the library through ctypes.  It mirrors the reference's
+operator surface: ``initLBM`` -> :class:`Lattice` construction,
+``stepLBMCPU`` -> :meth:`Lattice.step`, the ``lbmRho/lbmUx/...`` globals ->
+:meth:`Lattice.field`, ``lbmControlVolumeForce`` -> :meth:`Lattice.cv_force`
+(aeros/src/lbm.h:156-202 of the reference).
+
+Importing this module needs no GPU; creating a :class:`Lattice` does, and
+fails loudly if the HIP library is missing or no device is usable.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "build" / "libaeros_lbm.so"
+
+FP32, FP64, FP16, BF16 = 0, 1, 2, 3
+PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp16": FP16, "bf16": BF16}
+
+(FIELD_RHO, FIELD_UX, FIELD_UY, FIELD_UZ, FIELD_UX_WORLD, FIELD_UY_WORLD, FIELD_UZ_WORLD,
+ FIELD_PRESSURE, FIELD_TKE, FIELD_STRAIN, FIELD_SOLID, FIELD_GROUND, FIELD_WALLDIST,
+ FIELD_T, FIELD_BUOYANCY) = range(15)
+FIELD_NAMES = {
+    "rho": FIELD_RHO, "ux": FIELD_UX, "uy": FIELD_UY, "uz": FIELD_UZ,
+    "ux_world": FIELD_UX_WORLD, "uy_world": FIELD_UY_WORLD, "uz_world": FIELD_UZ_WORLD,
+    "pressure": FIELD_PRESSURE, "tke": FIELD_TKE, "strain": FIELD_STRAIN, "solid": FIELD_SOLID,
+    "ground": FIELD_GROUND, "walldist": FIELD_WALLDIST, "T": FIELD_T, "buoyancy": FIELD_BUOYANCY,
+}
+
+ERRORS = {0: "ok", -1: "bad argument", -2: "out of memory", -3: "HIP error", -4: "bad state",
+          -5: "domain too large", -6: "unsupported", -7: "no device"}
+
+
+class Config(C.Structure):
+    """aeros_lbm_config: everything initLBM reads (aeros/src/lbm.cpp:405-886)."""
+    _fields_ = [
+        ("struct_size", C.c_int32),
+        ("voxels", C.POINTER(C.c_int32)),
+        ("vox_n", C.c_int32 * 3),
+        ("vox_min", C.c_float * 3),
+        ("vox_max", C.c_float * 3),
+        ("voxel_resolution", C.c_int32),
+        ("flow_speed", C.c_float),
+        ("flow_azimuth_deg", C.c_float),
+        ("flow_elevation_deg", C.c_float),
+        ("air_density", C.c_float),
+        ("air_temperature", C.c_float),
+        ("max_dim", C.c_float),
+        ("frontal_area", C.c_float),
+        ("ground_effect", C.c_int32),
+        ("ground_effect_height", C.c_float),
+        ("tau", C.c_float),
+        ("U0", C.c_float),
+        ("smagorinsky_c", C.c_float),
+        ("convergence_threshold", C.c_float),
+        ("domain_padding", C.c_float),
+        ("inlet_turbulence", C.c_float),
+        ("ground_height", C.c_float),
+        ("body_force", C.c_float * 3),
+        ("use_les", C.c_int32),
+        ("use_trt", C.c_int32),
+        ("convective_outlet", C.c_int32),
+        ("periodic_x", C.c_int32),
+        ("periodic_z", C.c_int32),
+        ("auto_viscosity", C.c_int32),
+        ("use_ground", C.c_int32),
+        ("thermal", C.c_int32),
+        ("rayleigh", C.c_float),
+        ("prandtl", C.c_float),
+        ("fixed_wall_temps", C.c_int32),
+        ("hot_temp", C.c_float),
+        ("cold_temp", C.c_float),
+        ("perturbation", C.c_float),
+        ("precision", C.c_int32),
+        ("cell_cap", C.c_int64),
+        ("device", C.c_int32),
+    ]
+
+
+class Scalars(C.Structure):
+    """aeros_lbm_scalars: the reference's scalar globals (aeros/src/lbm.h:77-125)."""
+    _fields_ = [
+        ("step", C.c_int32), ("converged", C.c_int32), ("resets", C.c_int32),
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("avg_rho", C.c_float), ("avg_ke", C.c_float), ("max_u_lb", C.c_float),
+        ("max_u_world", C.c_float), ("tke", C.c_float), ("convergence", C.c_float),
+        ("neg_pops", C.c_int64), ("bad_pops", C.c_int64), ("rho_out_of_range", C.c_int64),
+        ("rho_non_finite", C.c_int64), ("u_limited", C.c_int64), ("pressure_non_finite", C.c_int64),
+        ("tau", C.c_float), ("viscosity", C.c_float), ("U0", C.c_float),
+        ("reynolds_phys", C.c_float), ("reynolds_lb", C.c_float), ("blockage", C.c_float),
+        ("pressure_min", C.c_float), ("pressure_max", C.c_float),
+        ("residual_rho", C.c_float), ("residual_u", C.c_float), ("time_ms", C.c_float),
+        ("fluid_cells", C.c_int64),
+        ("nusselt", C.c_float), ("thermal_mean_t", C.c_float), ("thermal_alpha", C.c_float),
+        ("thermal_gravity", C.c_float), ("thermal_tau", C.c_float), ("thermal_rayleigh", C.c_float),
+        ("thermal_prandtl", C.c_float), ("thermal_ready", C.c_int32),
+        ("thermal_ylo", C.c_int32), ("thermal_yhi", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BackendInfo(C.Structure):
+    _fields_ = [
+        ("compiled_with_hip", C.c_int32), ("device_count", C.c_int32), ("usable", C.c_int32),
+        ("gfx_major", C.c_int32), ("gfx_minor", C.c_int32), ("gfx_stepping", C.c_int32),
+        ("runtime_version", C.c_int32), ("driver_version", C.c_int32),
+        ("total_memory_mb", C.c_int64), ("compute_units", C.c_int32), ("last_error", C.c_int32),
+        ("device_name", C.c_char * 128), ("arch_name", C.c_char * 64), ("status", C.c_char * 512),
+    ]
+
+
+def default_config(**overrides) -> Config:
+    """Reference defaults (LBMParams, ThermalParams, globals), then overrides.
+
+    Mirrors aeros_lbm_config_defaults without needing the library loaded."""
+    c = Config()
+    c.struct_size = C.sizeof(Config)
+    c.voxel_resolution = 48
+    for i in range(3):
+        c.vox_min[i] = -1.0
+        c.vox_max[i] = 1.0
+    c.flow_speed = 2.0
+    c.air_density = 1.225
+    c.air_temperature = 288.15
+    c.max_dim = 1.0
+    c.tau = 0.6
+    c.U0 = 0.1
+    c.smagorinsky_c = 0.12
+    c.convergence_threshold = 1e-5
+    c.domain_padding = 0.25
+    c.inlet_turbulence = 0.02
+    c.use_les = 1
+    c.convective_outlet = 1
+    c.auto_viscosity = 1
+    c.rayleigh = 1.0e5
+    c.prandtl = 0.71
+    c.fixed_wall_temps = 1
+    c.hot_temp = 1.0
+    c.cold_temp = 0.0
+    c.perturbation = 0.02
+    c.precision = FP32
+    apply_overrides(c, overrides)
+    return c
+
+
+def apply_overrides(c: Config, overrides: dict) -> Config:
+    for k, v in overrides.items():
+        if k == "voxels":
+            set_voxels(c, v)
+        elif k in ("vox_n", "vox_min", "vox_max", "body_force"):
+            arr = getattr(c, k)
+            for i in range(3):
+                arr[i] = v[i]
+        elif k == "precision" and isinstance(v, str):
+            c.precision = PRECISIONS[v]
+        else:
+            if not hasattr(c, k):
+                raise AttributeError(f"unknown config field {k}")
+            setattr(c, k, v)
+    return c
+
+
+def set_voxels(c: Config, vox) -> None:
+    """Attach a voxel grid (z, y, x) int32 array; the array is kept alive on the struct."""
+    if vox is None:
+        c.voxels = None
+        c._vox_keep = None
+        return
+    vox = np.ascontiguousarray(vox, dtype=np.int32)
+    nz, ny, nx = vox.shape
+    c.vox_n[0], c.vox_n[1], c.vox_n[2] = nx, ny, nz
+    c.voxels = vox.ctypes.data_as(C.POINTER(C.c_int32))
+    c._vox_keep = vox
+
+
+_lib = None
+
+
+def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
+    """Load libaeros_lbm.so and declare its prototypes.  Raises if it is missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"HIP library {p} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(str(p))
+    P = C.POINTER
+    vp = C.c_void_p
+    sig = {
+        "aeros_lbm_config_defaults": (None, [P(Config)]),
+        "aeros_lbm_create": (vp, [P(Config), P(C.c_int)]),
+        "aeros_lbm_destroy": (C.c_int, [vp]),
+        "aeros_lbm_reset": (C.c_int, [vp]),
+        "aeros_lbm_dims": (C.c_int, [vp, P(C.c_int32), P(C.c_float), P(C.c_float)]),
+        "aeros_lbm_set_mask": (C.c_int, [vp, P(C.c_uint8), C.c_int32, C.c_int32, C.c_int32]),
+        "aeros_lbm_step": (C.c_int, [vp, C.c_int32]),
+        "aeros_lbm_update": (C.c_int, [vp, C.c_float, C.c_int32, C.c_int32]),
+        "aeros_lbm_read_field": (C.c_int, [vp, C.c_int32, P(C.c_float)]),
+        "aeros_lbm_read_field_f64": (C.c_int, [vp, C.c_int32, P(C.c_double)]),
+        "aeros_lbm_read": (C.c_int, [vp, C.c_uint32, P(C.c_float), P(C.c_float), P(C.c_float),
+                                     P(C.c_float), P(C.c_float)]),
+        "aeros_lbm_get_scalars": (C.c_int, [vp, P(Scalars)]),
+        "aeros_lbm_read_populations": (C.c_int, [vp, P(C.c_double)]),
+        "aeros_lbm_cv_force": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, P(C.c_double)]),
+        "aeros_lbm_projected_area": (C.c_int, [vp, C.c_int32, P(C.c_float)]),
+        "aeros_lbm_checksums": (C.c_int, [vp, P(C.c_double), P(C.c_double)]),
+        "aeros_lbm_thermal_upload": (C.c_int, [vp, P(C.c_float)]),
+        "aeros_lbm_thermal_reset": (C.c_int, [vp]),
+        "aeros_lbm_equilibrium_moments": (C.c_int, [C.c_float] * 4 + [P(C.c_float)] * 4),
+        "aeros_voxelize": (C.c_int, [P(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                     P(C.c_float), P(C.c_float), P(C.c_int32)]),
+        "aeros_voxel_plan": (C.c_int, [P(C.c_float), P(C.c_float), C.c_float, C.c_int32,
+                                       P(C.c_int32), P(C.c_float), P(C.c_float)]),
+        "aeros_lbm_last_error": (C.c_int, [C.c_char_p, C.c_int32]),
+        "aeros_probe": (C.c_int, [P(BackendInfo)]),
+        "aeros_release_all": (None, []),
+        "aeros_lbm_kernel_timing": (C.c_int, [vp, C.c_int32, P(C.c_double), P(C.c_int64)]),
+        "setVoxelData": (None, [P(C.c_int), P(C.c_float), C.c_int, C.c_int, C.c_int] + [C.c_float] * 6),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib if path is None else _lib
+    return lib
+
+
+def exported_symbols() -> list[str]:
+    """Names of every entry point include/aeros_lbm.h declares."""
+    hdr = (HERE.parent / "include" / "aeros_lbm.h").read_text()
+    import re
+    names = re.findall(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b(aeros_\w+|setVoxelData)\s*\(", hdr, re.M)
+    return sorted(set(names))
+
+
+def last_error() -> str:
+    lib = load_library()
+    buf = C.create_string_buffer(1024)
+    code = lib.aeros_lbm_last_error(buf, 1024)
+    return f"[{code}] {buf.value.decode(errors='replace')}" if code else ""
+
+
+class AerosError(RuntimeError):
+    pass
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise AerosError(f"{what} failed: {ERRORS.get(rc, rc)} {last_error()}")
+
+
+def probe() -> dict:
+    lib = load_library()
+    info = BackendInfo()
+    lib.aeros_probe(C.byref(info))
+    d = {k: getattr(info, k) for k, _ in info._fields_}
+    for k in ("device_name", "arch_name", "status"):
+        d[k] = d[k].decode(errors="replace")
+    return d
+
+
+class Lattice:
+    """One solver instance on the GPU (the reference's initLBM .. shutdownLBM)."""
+
+    def __init__(self, config: Config | None = None, **overrides):
+        self.lib = load_library()
+        cfg = config if config is not None else default_config()
+        if overrides:
+            apply_overrides(cfg, overrides)
+        self.config = cfg
+        err = C.c_int(0)
+        h = self.lib.aeros_lbm_create(C.byref(cfg), C.byref(err))
+        if not h:
+            raise AerosError(f"aeros_lbm_create failed: {ERRORS.get(err.value, err.value)} {last_error()}")
+        self.h = C.c_void_p(h)
+        d = (C.c_int32 * 3)()
+        mn = (C.c_float * 3)()
+        cs = (C.c_float * 3)()
+        _check(self.lib.aeros_lbm_dims(self.h, d, mn, cs), "aeros_lbm_dims")
+        self.dims = (d[0], d[1], d[2])
+        self.box_min = tuple(mn)
+        self.cell_size = tuple(cs)
+
+    @property
+    def shape(self):
+        nx, ny, nz = self.dims
+        return (nz, ny, nx)
+
+    @property
+    def ncells(self):
+        nx, ny, nz = self.dims
+        return nx * ny * nz
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.aeros_lbm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, n: int = 1):
+        _check(self.lib.aeros_lbm_step(self.h, int(n)), "aeros_lbm_step")
+
+    def update(self, dt: float, force_steps: int = -1, steps_per_frame: int = 3):
+        _check(self.lib.aeros_lbm_update(self.h, float(dt), int(force_steps), int(steps_per_frame)),
+               "aeros_lbm_update")
+
+    def reset(self):
+        _check(self.lib.aeros_lbm_reset(self.h), "aeros_lbm_reset")
+
+    def set_mask(self, solid):
+        solid = np.ascontiguousarray(solid, dtype=np.uint8)
+        nz, ny, nx = solid.shape
+        _check(self.lib.aeros_lbm_set_mask(self.h, solid.ctypes.data_as(C.POINTER(C.c_uint8)), nx, ny, nz),
+               "aeros_lbm_set_mask")
+
+    def field(self, name, dtype=np.float32):
+        fid = FIELD_NAMES[name] if isinstance(name, str) else int(name)
+        if dtype == np.float64:
+            out = np.empty(self.ncells, dtype=np.float64)
+            _check(self.lib.aeros_lbm_read_field_f64(self.h, fid, out.ctypes.data_as(C.POINTER(C.c_double))),
+                   "aeros_lbm_read_field_f64")
+        else:
+            out = np.empty(self.ncells, dtype=np.float32)
+            _check(self.lib.aeros_lbm_read_field(self.h, fid, out.ctypes.data_as(C.POINTER(C.c_float))),
+                   "aeros_lbm_read_field")
+        return out.reshape(self.shape)
+
+    def populations(self):
+        out = np.empty(self.ncells * 19, dtype=np.float64)
+        _check(self.lib.aeros_lbm_read_populations(self.h, out.ctypes.data_as(C.POINTER(C.c_double))),
+               "aeros_lbm_read_populations")
+        return out.reshape(self.shape + (19,))
+
+    def scalars(self) -> dict:
+        s = Scalars()
+        _check(self.lib.aeros_lbm_get_scalars(self.h, C.byref(s)), "aeros_lbm_get_scalars")
+        return s.as_dict()
+
+    def cv_force(self, hx=0.30, hy=0.30, hz=0.30):
+        out = (C.c_double * 3)()
+        _check(self.lib.aeros_lbm_cv_force(self.h, hx, hy, hz, out), "aeros_lbm_cv_force")
+        return tuple(out)
+
+    def projected_area(self, axis=0):
+        v = C.c_float()
+        _check(self.lib.aeros_lbm_projected_area(self.h, axis, C.byref(v)), "aeros_lbm_projected_area")
+        return v.value
+
+    def checksums(self):
+        a, b = C.c_double(), C.c_double()
+        _check(self.lib.aeros_lbm_checksums(self.h, C.byref(a), C.byref(b)), "aeros_lbm_checksums")
+        return a.value, b.value
+
+    def thermal_upload(self, T):
+        T = np.ascontiguousarray(T, dtype=np.float32).ravel()
+        if T.size != self.ncells:
+            raise ValueError("temperature field size mismatch")
+        _check(self.lib.aeros_lbm_thermal_upload(self.h, T.ctypes.data_as(C.POINTER(C.c_float))),
+               "aeros_lbm_thermal_upload")
+
+    def thermal_reset(self):
+        _check(self.lib.aeros_lbm_thermal_reset(self.h), "aeros_lbm_thermal_reset")
+
+    def kernel_timing(self, enable: int):
+        """enable 1/0/-1 (start/stop/reset); returns (total ms, launches) of the step kernel."""
+        ms, n = C.c_double(), C.c_int64()
+        _check(self.lib.aeros_lbm_kernel_timing(self.h, int(enable), C.byref(ms), C.byref(n)),
+               "aeros_lbm_kernel_timing")
+        return ms.value, n.value
+
+
+def equilibrium_moments(rho, ux, uy, uz):
+    lib = load_library()
+    o = [C.c_float() for _ in range(4)]
+    ok = lib.aeros_lbm_equilibrium_moments(rho, ux, uy, uz, *[C.byref(v) for v in o])
+    return bool(ok), tuple(v.value for v in o)
+
+
+def voxelize(tris, dims, box_min, cell):
+    """BVH three-ray vote at cell centres; tris is (T, 9) float32. Returns (nz, ny, nx) int32."""
+    lib = load_library()
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    nx, ny, nz = dims
+    out = np.empty(nx * ny * nz, dtype=np.int32)
+    mn = (C.c_float * 3)(*box_min)
+    cs = (C.c_float * 3)(*cell)
+    _check(lib.aeros_voxelize(tris.ctypes.data_as(C.POINTER(C.c_float)), tris.shape[0], nx, ny, nz, mn, cs,
+                              out.ctypes.data_as(C.POINTER(C.c_int32))), "aeros_voxelize")
+    return out.reshape(nz, ny, nx)

@@ -1,0 +1,99 @@
+// Shared declarations of the MI355X LBM library: host-side setup results and
+// the parameter blocks handed to the HIP kernels by value.
+#pragma once
+
+#include <stdint.h>
+#include <vector>
+
+#include "../../include/aeros_lbm.h"
+
+namespace aeros {
+
+constexpr int Q = 19;   // D3Q19
+constexpr int Q7 = 7;   // D3Q7 (first seven D3Q19 directions)
+
+// Per-cell link mask (uint32), built once per mask:
+//   bits 0..18  source of direction q is solid/ground (half-way bounce-back);
+//               for shell cells the bit refers to the wrapped in-domain source
+//   bit  19     the cell itself is solid or ground (never updated)
+//   bits 20..22 Van Driest table index: wall distance 0..5, 6 = no damping
+//   bit  23     Smagorinsky strain is defined here (interior, not on the inlet
+//               or outlet plane, no solid 6-neighbour)
+//   bit  24     inlet plane (geometric)      bit 25  outlet plane (geometric)
+//   bit  26     shell cell: the general pull (wrap / out-of-domain copy) applies
+//   bit  27     pull = inlet equilibrium     bit 28  pull = convective outlet
+constexpr uint32_t M_SOLID = 1u << 19;
+constexpr int M_VD_SHIFT = 20;
+constexpr uint32_t M_VD_MASK = 7u << 20;
+constexpr uint32_t M_STRAIN = 1u << 23;
+constexpr uint32_t M_INLET = 1u << 24;
+constexpr uint32_t M_OUTLET = 1u << 25;
+constexpr uint32_t M_SHELL = 1u << 26;
+constexpr uint32_t M_PULL_INLET = 1u << 27;
+constexpr uint32_t M_PULL_OUTLET = 1u << 28;
+constexpr uint32_t M_LINKS = (1u << 19) - 1u;
+
+// Result of the host-side part of initLBM (domain, mask, wall distance, tau).
+struct Setup {
+    aeros_lbm_config cfg{};
+    int nx = 0, ny = 0, nz = 0;
+    int64_t total = 0;
+    float min[3] = {0, 0, 0};
+    float cell[3] = {0.1f, 0.1f, 0.1f};
+    std::vector<uint8_t> solid, ground, wall;   // per cell
+    int64_t fluid = 1;
+    float inlet[3] = {0, 0, 0};                 // computeInletVelocityLB
+    int axis = 0, sign = 1;                     // computeFlowAxis
+    float tau = 0.6f, viscosity = 0.033f, U0 = 0.1f;
+    float re_lb = 0, re_phys = 0, blockage = 0;
+    // thermal (thermalInit)
+    bool thermal_ready = false;
+    int ylo = 1, yhi = 0;
+    float H = 1, tauT = 0.7f, alpha = 0, gravity = 0, rayleigh = 0, prandtl = 0;
+    bool curl_noise = false;                    // inletTurbulence > 0.001
+};
+
+// initLBM's host half: returns AEROS_OK or an error code; msg gets the reason.
+int setup_domain(const aeros_lbm_config& cfg, Setup& s, char* msg, int msglen);
+// Everything after the mask is known: wall distance, fluid count, tau, thermal.
+int setup_after_mask(Setup& s, char* msg, int msglen);
+void config_defaults(aeros_lbm_config* c);
+void inlet_velocity(const aeros_lbm_config& c, float& ux, float& uy, float& uz);
+void flow_axis(const aeros_lbm_config& c, int& axis, int& sign);
+// Initial temperature field of thermalResetField (host, glibc cos/sin).
+void thermal_initial_field(const Setup& s, std::vector<float>& T);
+// Initial per-cell velocity with the curl-noise inflow (host, glibc cos).
+void curl_noise_velocity(const Setup& s, std::vector<float>& u3);
+
+// Voxelizer (BVH three-ray vote), host side.
+int voxelize_host(const float* tris, int64_t ntris, int nx, int ny, int nz, const float mn[3],
+                  const float cs[3], int32_t* out);
+
+// ----------------------------------------------------------------- kernels
+template <class R> struct StepParams {
+    int nx, ny, nz;
+    int64_t n;
+    int axis, sign;          // flow axis / sign
+    int les, trt, thermal, periodic_x, periodic_z, conv_outlet;
+    int first;               // state holds pre-collision populations: collide only
+    int write_fields;        // write rho/u/tke/strain fields this step
+    int dry;                 // re-run of an aborted step: no abort check, no reduction
+    R tau0, cs, vel_scale2;  // sanitised tau, Smagorinsky constant, (flowSpeed/U0)^2
+    R vd[7];                 // Van Driest table
+    R fx, fy, fz;            // body force
+    R in_u[3];               // inlet velocity
+    R feq_in[Q];             // computeEquilibriumFast(q, 1, u_in)
+    R conv_c, conv_k2;       // convective outlet c and 0.15 c
+    int64_t out_d1, out_d2;  // cell offsets to the planes N-2, N-3 behind the outlet
+    // thermal
+    R t_inv_tau, t_ref, t_g, t_hi, t_lo;
+    R t_hot, t_cold;
+    int t_fixed;
+    // reduction
+    int64_t fluid_cells;
+    int64_t nan_limit;       // total / 10
+    float vel_scale;         // flowSpeed / U0 (clamped), for max |u| world
+    float conv_threshold;
+};
+
+}  // namespace aeros

@@ -1,0 +1,928 @@
+// HIP kernels of the MI355X (gfx950) lattice Boltzmann solver.
+//
+// One fused pull kernel per lattice step: every thread owns one cell, pulls
+// the 19 post-collision populations its neighbours stored in the previous
+// step (half-way bounce-back from its own opposite population where the
+// source is solid, wrap / copy-in-place on the domain shell, equilibrium on
+// the inlet, the convective rule on the outlet), collides them (BGK/TRT, Guo
+// forcing, Smagorinsky with Van Driest damping from the previous step's
+// velocity), then advances the D3Q7 temperature lattice with the fresh
+// velocity and writes everything back.  Populations live in HBM as
+// structure-of-arrays f[q][cell] so a wave's 64 lanes touch one contiguous
+// 256-byte (FP32) row per direction.
+//
+// Arithmetic follows the reference's stepLBMCPU / thermalStep operation by
+// operation (aeros/src/lbm.cpp:907-1709, aeros/src/thermal_lbm.cpp:199-355):
+// built with -ffp-contract=off, correctly rounded division and square root,
+// so the FP32 path reproduces the reference's float results bit for bit.
+// The diagnostics of every step are reduced in a fixed order (per-thread,
+// then an LDS tree, then the last block folds the block partials in index
+// order), so results do not depend on scheduling.
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+
+#include "lbm_common.h"
+
+namespace aeros {
+
+// ------------------------------------------------------------ lattice tables
+// D3Q19 in the reference's order (aeros/src/lbm.cpp:92-109).
+__device__ __host__ constexpr int cx_of(int q) {
+    return q == 1 || q == 7 || q == 9 || q == 11 || q == 13 ? 1 : (q == 2 || q == 8 || q == 10 || q == 12 || q == 14 ? -1 : 0);
+}
+__device__ __host__ constexpr int cy_of(int q) {
+    return q == 3 || q == 7 || q == 10 || q == 15 || q == 17 ? 1 : (q == 4 || q == 8 || q == 9 || q == 16 || q == 18 ? -1 : 0);
+}
+__device__ __host__ constexpr int cz_of(int q) {
+    return q == 5 || q == 11 || q == 14 || q == 15 || q == 18 ? 1 : (q == 6 || q == 12 || q == 13 || q == 16 || q == 17 ? -1 : 0);
+}
+__device__ __host__ constexpr int opp_of(int q) { return q == 0 ? 0 : (q & 1 ? q + 1 : q - 1); }
+
+template <class R> __device__ __host__ inline R w_of(int q) {
+    return q == 0 ? R(1) / R(3) : (q <= 6 ? R(1) / R(18) : R(1) / R(36));
+}
+template <class R> __device__ __host__ inline R w7_of(int q) { return q == 0 ? R(0.25) : R(0.125); }
+
+// ------------------------------------------------------------ storage
+struct bf16_t { uint16_t bits; };
+
+template <class S, class R> struct Pop;
+template <> struct Pop<float, float> {
+    __device__ static float ld(const float* p, int64_t i, float) { return p[i]; }
+    __device__ static void st(float* p, int64_t i, float v, float) { p[i] = v; }
+};
+template <> struct Pop<double, double> {
+    __device__ static double ld(const double* p, int64_t i, double) { return p[i]; }
+    __device__ static void st(double* p, int64_t i, double v, double) { p[i] = v; }
+};
+template <> struct Pop<_Float16, float> {   // deviation from w_q, IEEE half, round to nearest even
+    __device__ static float ld(const _Float16* p, int64_t i, float w) { return w + (float)p[i]; }
+    __device__ static void st(_Float16* p, int64_t i, float v, float w) { p[i] = (_Float16)(v - w); }
+};
+__device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ inline uint16_t f_to_bf16(float x) {   // round to nearest even
+    uint32_t u = __float_as_uint(x);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)(u >> 16);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+template <> struct Pop<bf16_t, float> {     // deviation from w_q, bfloat16
+    __device__ static float ld(const bf16_t* p, int64_t i, float w) { return w + bf16_to_f(p[i].bits); }
+    __device__ static void st(bf16_t* p, int64_t i, float v, float w) { p[i].bits = f_to_bf16(v - w); }
+};
+
+template <class U, class R> struct Vel;
+template <> struct Vel<float, float> {
+    __device__ static float ld(const float* p, int64_t i) { return p[i]; }
+    __device__ static void st(float* p, int64_t i, float v) { p[i] = v; }
+};
+template <> struct Vel<double, double> {
+    __device__ static double ld(const double* p, int64_t i) { return p[i]; }
+    __device__ static void st(double* p, int64_t i, double v) { p[i] = v; }
+};
+template <> struct Vel<_Float16, float> {
+    __device__ static float ld(const _Float16* p, int64_t i) { return (float)p[i]; }
+    __device__ static void st(_Float16* p, int64_t i, float v) { p[i] = (_Float16)v; }
+};
+
+template <class R> __device__ inline bool fin(R x) { return __builtin_isfinite(x); }
+
+// ------------------------------------------------------------ equilibria
+// equilibriumKernel (aeros/src/lbm.cpp:141-144).  cu is formed as the
+// reference does, (cx*ux + cy*uy) + cz*uz with cx,cy,cz in {-1,0,1}; the
+// products by 0 and 1 are exact, so the sum of the signed nonzero terms in
+// the same order gives the same bits.
+template <class R, int q> __device__ inline R cdotu(R ux, R uy, R uz) {
+    constexpr int a = cx_of(q), b = cy_of(q), c = cz_of(q);
+    R s = R(0);
+    bool have = false;
+    if (a != 0) { s = a > 0 ? ux : -ux; have = true; }
+    if (b != 0) { const R t = b > 0 ? uy : -uy; s = have ? s + t : t; have = true; }
+    if (c != 0) { const R t = c > 0 ? uz : -uz; s = have ? s + t : t; have = true; }
+    return s;
+}
+template <class R, int q> __device__ inline R feq_k(R rho, R ux, R uy, R uz, R usqr) {
+    const R cu = cdotu<R, q>(ux, uy, uz);
+    return w_of<R>(q) * rho * (R(1) + cu * R(3) + cu * cu * R(4.5) - usqr * R(1.5));
+}
+// computeEquilibriumFast (aeros/src/lbm.cpp:146-184), guarded form
+template <class R> __device__ inline R feq_guarded(int q, R rho, R ux, R uy, R uz, R usqr) {
+    if (!fin(rho)) rho = R(1);
+    if (usqr > R(0.09)) {
+        const R s = __builtin_sqrt(R(0.09) / usqr);
+        ux *= s; uy *= s; uz *= s;
+        usqr = ux * ux + uy * uy + uz * uz;
+    }
+    R cu = (R)cx_of(q) * ux + (R)cy_of(q) * uy + (R)cz_of(q) * uz;
+    if (!fin(cu)) cu = R(0);
+    R f = w_of<R>(q) * rho * (R(1) + cu * R(3) + cu * cu * R(4.5) - usqr * R(1.5));
+    if (!fin(f)) f = w_of<R>(q) * rho;
+    return f;
+}
+
+// Guo source term 3 w [(c - u_e) + 3 (c.u_e) c] . F (aeros/src/lbm.cpp:1213-1217)
+template <class R, int q> __device__ inline R guo(R ex, R ey, R ez, R Fx, R Fy, R Fz) {
+    const R cx = (R)cx_of(q), cy = (R)cy_of(q), cz = (R)cz_of(q);
+    const R cu = cx * ex + cy * ey + cz * ez;
+    return R(3) * w_of<R>(q) * ((cx - ex + R(3) * cu * cx) * Fx + (cy - ey + R(3) * cu * cy) * Fy +
+                                (cz - ez + R(3) * cu * cz) * Fz);
+}
+
+// ------------------------------------------------------------ reductions
+struct Partial {
+    double rho, ke, tke;
+    float maxv;
+    int32_t nan, neg, bad, rho_out, rho_nf, ulim;
+};
+
+// Device-resident scalar state, updated by the last block of every step
+// kernel exactly as the reference updates its globals
+// (aeros/src/lbm.cpp:1313-1319, 1554-1580).
+struct Status {
+    int32_t step;            // lbmCurrentStep
+    int32_t abort;           // 0, 1 = NaN storm (step not counted), 2 = divergence (step counted)
+    int32_t abort_step;      // reference step number at which the loop broke
+    int32_t converged;
+    float avg_rho, avg_ke, max_u_lb, max_u_world, tke, convergence, prev_ke;
+    int32_t pad0;
+    int64_t neg, bad, rho_out, rho_nf, ulim;
+    uint32_t ticket;         // last-block counter
+    int32_t pad1;
+};
+
+constexpr int BLOCK = 256;
+
+template <class T> __device__ inline void lds_tree_sum(T* s, int tid) {
+    for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        __syncthreads();
+        if (tid < off) s[tid] += s[tid + off];
+    }
+    __syncthreads();
+}
+__device__ inline void lds_tree_max(float* s, int tid) {
+    for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        __syncthreads();
+        if (tid < off) s[tid] = fmaxf(s[tid], s[tid + off]);
+    }
+    __syncthreads();
+}
+
+struct RedLds {
+    double d[3][BLOCK];
+    float m[BLOCK];
+    int32_t i[6][BLOCK];
+};
+
+__device__ inline void block_reduce(RedLds& L, int tid, Partial& p) {
+    L.d[0][tid] = p.rho; L.d[1][tid] = p.ke; L.d[2][tid] = p.tke; L.m[tid] = p.maxv;
+    L.i[0][tid] = p.nan; L.i[1][tid] = p.neg; L.i[2][tid] = p.bad; L.i[3][tid] = p.rho_out;
+    L.i[4][tid] = p.rho_nf; L.i[5][tid] = p.ulim;
+    for (int k = 0; k < 3; ++k) lds_tree_sum(L.d[k], tid);
+    lds_tree_max(L.m, tid);
+    for (int k = 0; k < 6; ++k) lds_tree_sum(L.i[k], tid);
+    p.rho = L.d[0][0]; p.ke = L.d[1][0]; p.tke = L.d[2][0]; p.maxv = L.m[0];
+    p.nan = L.i[0][0]; p.neg = L.i[1][0]; p.bad = L.i[2][0]; p.rho_out = L.i[3][0];
+    p.rho_nf = L.i[4][0]; p.ulim = L.i[5][0];
+}
+
+// The last block to finish folds the per-block partials in block order and
+// applies the reference's end-of-step logic.
+template <class R>
+__device__ void finalize_step(const StepParams<R>& P, Partial* parts, Status* st, RedLds& L, int tid) {
+    const int nb = gridDim.x;
+    double a = 0, b = 0, c = 0;
+    float m = 0.0f;
+    int64_t n0 = 0, n1 = 0, n2 = 0, n3 = 0, n4 = 0, n5 = 0;
+    for (int k = tid; k < nb; k += BLOCK) {   // fixed assignment of partials to threads
+        const Partial& q = parts[k];
+        a += q.rho; b += q.ke; c += q.tke; m = fmaxf(m, q.maxv);
+        n0 += q.nan; n1 += q.neg; n2 += q.bad; n3 += q.rho_out; n4 += q.rho_nf; n5 += q.ulim;
+    }
+    // int64 counters: reuse the double slots bitwise-free via two passes
+    L.d[0][tid] = a; L.d[1][tid] = b; L.d[2][tid] = c; L.m[tid] = m;
+    for (int k = 0; k < 3; ++k) lds_tree_sum(L.d[k], tid);
+    lds_tree_max(L.m, tid);
+    const double sr = L.d[0][0], sk = L.d[1][0], stk = L.d[2][0];
+    const float mv = L.m[0];
+    __syncthreads();
+    int64_t* li = reinterpret_cast<int64_t*>(&L.d[0][0]);   // 3*BLOCK doubles = room for 6 x BLOCK/2... use two rounds
+    int64_t cnt[6] = {n0, n1, n2, n3, n4, n5};
+    int64_t tot[6];
+    for (int r = 0; r < 2; ++r) {
+        for (int k = 0; k < 3; ++k) li[k * BLOCK + tid] = cnt[r * 3 + k];
+        for (int k = 0; k < 3; ++k) {
+            int64_t* s = li + k * BLOCK;
+            for (int off = BLOCK / 2; off > 0; off >>= 1) {
+                __syncthreads();
+                if (tid < off) s[tid] += s[tid + off];
+            }
+        }
+        __syncthreads();
+        for (int k = 0; k < 3; ++k) tot[r * 3 + k] = li[k * BLOCK];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        if (tot[0] > P.nan_limit) {           // NaN storm: break before the stream
+            st->abort = 1;
+            st->abort_step = st->step + 1;
+        } else {
+            st->step += 1;
+            const float avg_rho = (float)(sr / (double)P.fluid_cells);
+            const float avg_ke = (float)(sk / (double)P.fluid_cells);
+            st->avg_rho = avg_rho;
+            st->avg_ke = avg_ke;
+            st->max_u_lb = mv;
+            st->tke = (float)(stk / (double)P.fluid_cells);
+            st->max_u_world = mv * P.vel_scale;
+            st->neg = tot[1]; st->bad = tot[2]; st->rho_out = tot[3]; st->rho_nf = tot[4]; st->ulim = tot[5];
+            st->convergence = fabsf(avg_ke - st->prev_ke);
+            st->prev_ke = avg_ke;
+            st->converged = st->convergence < P.conv_threshold;
+            if (!__builtin_isfinite(avg_rho) || avg_rho > 1.2f || avg_rho < 0.8f) {
+                st->abort = 2;
+                st->abort_step = st->step;
+            }
+        }
+        st->ticket = 0;
+    }
+}
+
+// ------------------------------------------------------------ the step kernel
+struct StepBuffers {
+    const void* f_in;
+    void* f_out;
+    const uint32_t* mask;
+    const void* u_prev[3];     // LES stencil input (previous step's velocity)
+    void* u_next[3];
+    void* fld[6];              // rho, ux, uy, uz, tke, strain (type R)
+    const void* g_in;          // thermal populations (type R)
+    void* g_out;
+    const void* b_in;          // buoyancy from the previous thermal step
+    void* b_out;
+    void* t_out;               // temperature
+    Partial* parts;
+    Status* status;
+};
+
+template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
+__global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) {
+    __shared__ RedLds L;
+    __shared__ int is_last;
+    Status* st = B.status;
+    if (!P.dry && st->abort) return;   // an earlier step of this call broke the loop
+
+    const S* fin_ = static_cast<const S*>(B.f_in);
+    S* fout = static_cast<S*>(B.f_out);
+    const int64_t n = P.n;
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const int tid = threadIdx.x;
+
+    Partial acc{0.0, 0.0, 0.0, 0.0f, 0, 0, 0, 0, 0, 0};
+
+    const int64_t stride = (int64_t)gridDim.x * BLOCK;
+    for (int64_t cell = (int64_t)blockIdx.x * BLOCK + tid; cell < n; cell += stride) {
+        const uint32_t m = B.mask[cell];
+        if (m & M_SOLID) continue;   // solid/ground: fields are constant (rho 1, u 0), never stored
+
+        // ---------------- pull S(x, q) from the previous post-collision state
+        R f[Q];
+        if (P.first) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) f[q] = Pop<S, R>::ld(fin_, q * n + cell, w_of<R>(q));
+        } else if (m & M_PULL_INLET) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) f[q] = P.feq_in[q];
+        } else if (m & M_PULL_OUTLET) {
+            // f_out + c (f_in - f_out) + 0.15 c (f_in - f_in2) on the planes N-1, N-2, N-3
+            // (aeros/src/lbm.cpp:1465-1544)
+            const int64_t ci = cell - P.out_d1, ci2 = cell - P.out_d2;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const R w = w_of<R>(q);
+                const R fo = Pop<S, R>::ld(fin_, q * n + cell, w);
+                const R fi = Pop<S, R>::ld(fin_, q * n + ci, w);
+                const R fi2 = Pop<S, R>::ld(fin_, q * n + ci2, w);
+                f[q] = fo + P.conv_c * (fi - fo) + P.conv_k2 * (fi - fi2);
+            }
+        } else if (!(m & M_SHELL)) {
+            // interior: every source is in the domain
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const bool bb = (m >> q) & 1u;
+                const int64_t src = bb ? cell : cell - (cx_of(q) + cy_of(q) * sy + cz_of(q) * sz);
+                const int dq = bb ? opp_of(q) : q;
+                f[q] = Pop<S, R>::ld(fin_, dq * n + src, w_of<R>(dq));
+            }
+        } else {
+            // domain shell: periodic wrap, out-of-domain sources copy the
+            // cell's own population (aeros/src/lbm.cpp:1362-1409)
+            const int x = (int)(cell % nx);
+            const int y = (int)((cell / nx) % ny);
+            const int z = (int)(cell / sz);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                int sx = x - cx_of(q), syy = y - cy_of(q), szz = z - cz_of(q);
+                if (P.periodic_x) { if (sx < 0) sx += nx; else if (sx >= nx) sx -= nx; }
+                if (P.periodic_z) { if (szz < 0) szz += nz; else if (szz >= nz) szz -= nz; }
+                const bool out = sx < 0 || sx >= nx || syy < 0 || syy >= ny || szz < 0 || szz >= nz;
+                const bool bb = !out && ((m >> q) & 1u);
+                const int64_t src = (out || bb) ? cell : ((int64_t)szz * ny + syy) * nx + sx;
+                const int dq = bb ? opp_of(q) : q;
+                f[q] = Pop<S, R>::ld(fin_, dq * n + src, w_of<R>(dq));
+            }
+        }
+
+        // ---------------- moments with sanitising (aeros/src/lbm.cpp:1061-1089)
+        R rho = R(0), ux = R(0), uy = R(0), uz = R(0);
+        int nanc = 0, badc = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            R fi = f[q];
+            if (!fin(fi)) { fi = w_of<R>(q); nanc++; badc++; }
+            if (fi < R(-1) || fi > R(10)) { fi = w_of<R>(q); badc++; }
+            // rho += f; u += f c with c in {-1,0,1}: the products by 0 only add
+            // signed zeros, so they are left out
+            rho += fi;
+            if (cx_of(q) > 0) ux += fi; else if (cx_of(q) < 0) ux -= fi;
+            if (cy_of(q) > 0) uy += fi; else if (cy_of(q) < 0) uy -= fi;
+            if (cz_of(q) > 0) uz += fi; else if (cz_of(q) < 0) uz -= fi;
+        }
+        if (!fin(rho)) { acc.rho_nf++; nanc++; rho = R(1); }
+        else if (rho < R(0.5) || rho > R(1.5)) { acc.rho_out++; nanc++; rho = R(1); }
+        {
+            const R inv = R(1) / rho;
+            ux *= inv; uy *= inv; uz *= inv;
+        }
+        if (!fin(ux)) { ux = uy = uz = R(0); nanc++; }
+        {
+            const R ut = ux * ux + uy * uy + uz * uz;
+            if (ut > R(0.09)) { const R s = __builtin_sqrt(R(0.09) / ut); ux *= s; uy *= s; uz *= s; }
+        }
+        const bool inlet = m & M_INLET, outlet = m & M_OUTLET;
+        if (inlet && !P.periodic_x) { rho = R(1); ux = P.in_u[0]; uy = P.in_u[1]; uz = P.in_u[2]; }
+
+        // ---------------- Smagorinsky from the previous step's velocity
+        // (aeros/src/lbm.cpp:979-1036, 1126-1145)
+        R tau_eff = P.tau0, smag = R(0);
+        if (LES && !inlet && !outlet) {
+            if (m & M_STRAIN) {
+                const U* ux0 = static_cast<const U*>(B.u_prev[0]);
+                const U* uy0 = static_cast<const U*>(B.u_prev[1]);
+                const U* uz0 = static_cast<const U*>(B.u_prev[2]);
+                const R h = R(0.5);
+                const R dux_dx = (Vel<U, R>::ld(ux0, cell + 1) - Vel<U, R>::ld(ux0, cell - 1)) * h;
+                const R duy_dy = (Vel<U, R>::ld(uy0, cell + sy) - Vel<U, R>::ld(uy0, cell - sy)) * h;
+                const R duz_dz = (Vel<U, R>::ld(uz0, cell + sz) - Vel<U, R>::ld(uz0, cell - sz)) * h;
+                const R dux_dy = (Vel<U, R>::ld(ux0, cell + sy) - Vel<U, R>::ld(ux0, cell - sy)) * h;
+                const R dux_dz = (Vel<U, R>::ld(ux0, cell + sz) - Vel<U, R>::ld(ux0, cell - sz)) * h;
+                const R duy_dx = (Vel<U, R>::ld(uy0, cell + 1) - Vel<U, R>::ld(uy0, cell - 1)) * h;
+                const R duy_dz = (Vel<U, R>::ld(uy0, cell + sz) - Vel<U, R>::ld(uy0, cell - sz)) * h;
+                const R duz_dx = (Vel<U, R>::ld(uz0, cell + 1) - Vel<U, R>::ld(uz0, cell - 1)) * h;
+                const R duz_dy = (Vel<U, R>::ld(uz0, cell + sy) - Vel<U, R>::ld(uz0, cell - sy)) * h;
+                const R sxy = h * (dux_dy + duy_dx), sxz = h * (dux_dz + duz_dx), syz = h * (duy_dz + duz_dy);
+                const R s2 = dux_dx * dux_dx + duy_dy * duy_dy + duz_dz * duz_dz + R(2) * (sxy * sxy + sxz * sxz + syz * syz);
+                smag = __builtin_sqrt(R(2) * s2);
+            }
+            if (fin(smag) && smag < R(10) && smag > R(1e-6)) {
+                R cs = P.cs;
+                const int vd = (int)((m & M_VD_MASK) >> M_VD_SHIFT);
+                if (vd <= 5) cs *= P.vd[vd];
+                const R nut = cs * cs * smag;
+                tau_eff = P.tau0 + nut / (R(1) / R(3));
+                if (tau_eff > R(1.5)) tau_eff = R(1.5);
+                if (tau_eff < R(0.51)) tau_eff = R(0.51);
+            }
+        }
+
+        R usqr = ux * ux + uy * uy + uz * uz;
+        if (usqr > R(0.09)) {
+            acc.ulim++;
+            const R s = __builtin_sqrt(R(0.09) / usqr);
+            ux *= s; uy *= s; uz *= s;
+            usqr = ux * ux + uy * uy + uz * uz;
+        }
+        const R inv_tau = R(1) / tau_eff;
+
+        // ---------------- Guo forcing (aeros/src/lbm.cpp:1161-1176)
+        R Fx = R(0), Fy = R(0), Fz = R(0);
+        if (FORCE) { Fx = P.fx; Fy = P.fy; Fz = P.fz; }
+        if (THERM) Fy += static_cast<const R*>(B.b_in)[cell];
+        const bool has_force = (FORCE || THERM) && (Fx != R(0) || Fy != R(0) || Fz != R(0));
+        R ex = ux, ey = uy, ez = uz, es = usqr;
+        if (has_force) {
+            const R hf = R(0.5) / rho;
+            ex += Fx * hf; ey += Fy * hf; ez += Fz * hf;
+            es = ex * ex + ey * ey + ez * ez;
+        }
+        R s_minus = inv_tau;
+        if (P.trt) {   // magic parameter 3/16 (aeros/src/lbm.cpp:1196-1202)
+            const R mm = inv_tau - R(0.5);
+            const R den = (__builtin_fabs(mm) > R(1e-6)) ? (R(0.1875) / mm + R(0.5)) : R(1);
+            s_minus = R(1) / (den > R(0.05) ? den : R(0.05));
+        }
+
+        // ---------------- pairwise TRT/BGK relaxation (aeros/src/lbm.cpp:1204-1264)
+        int negc = 0;
+        {
+            const R feq0 = feq_k<R, 0>(rho, ex, ey, ez, es);
+            R fi = f[0];
+            if (!fin(fi)) fi = feq0;
+            R fn = fi - (fi - feq0) * inv_tau;
+            if (has_force) fn = fn + (R(1) - R(0.5) * inv_tau) * guo<R, 0>(ex, ey, ez, Fx, Fy, Fz);
+            if (!fin(fn)) fn = feq0;
+            if (fn < R(0)) negc++;
+            f[0] = fn;
+        }
+        const R kp = R(1) - R(0.5) * inv_tau, km = R(1) - R(0.5) * s_minus;
+#define AEROS_PAIR(QI)                                                                             \
+        {                                                                                           \
+            constexpr int qi = QI, qo = QI + 1;                                                     \
+            const R feq = feq_k<R, qi>(rho, ex, ey, ez, es);                                        \
+            const R feqo = feq_k<R, qo>(rho, ex, ey, ez, es);                                       \
+            R fi = f[qi], fo = f[qo];                                                               \
+            if (!fin(fi)) fi = feq;                                                                 \
+            if (!fin(fo)) fo = feqo;                                                                \
+            const R fp = R(0.5) * (fi + fo), fm = R(0.5) * (fi - fo);                               \
+            const R ep = R(0.5) * (feq + feqo), em = R(0.5) * (feq - feqo);                         \
+            const R fpn = fp - inv_tau * (fp - ep);                                                 \
+            const R fmn = fm - s_minus * (fm - em);                                                 \
+            R ni, no;                                                                               \
+            if (has_force) {                                                                        \
+                const R si = guo<R, qi>(ex, ey, ez, Fx, Fy, Fz);                                    \
+                const R so = guo<R, qo>(ex, ey, ez, Fx, Fy, Fz);                                    \
+                const R sp = R(0.5) * (si + so), sm = R(0.5) * (si - so);                           \
+                ni = fpn + fmn + kp * sp + km * sm;                                                 \
+                no = fpn - fmn + kp * sp - km * sm;                                                 \
+            } else {                                                                                \
+                ni = fpn + fmn;                                                                     \
+                no = fpn - fmn;                                                                     \
+            }                                                                                       \
+            if (!fin(ni)) ni = feq;                                                                 \
+            if (!fin(no)) no = feqo;                                                                \
+            negc += (ni < R(0)) + (no < R(0));                                                      \
+            f[qi] = ni; f[qo] = no;                                                                 \
+        }
+        AEROS_PAIR(1) AEROS_PAIR(3) AEROS_PAIR(5) AEROS_PAIR(7) AEROS_PAIR(9)
+        AEROS_PAIR(11) AEROS_PAIR(13) AEROS_PAIR(15) AEROS_PAIR(17)
+#undef AEROS_PAIR
+
+#pragma unroll
+        for (int q = 0; q < Q; ++q) Pop<S, R>::st(fout, q * n + cell, f[q], w_of<R>(q));
+
+        // ---------------- velocity and diagnostics (aeros/src/lbm.cpp:1266-1305)
+        R vx = ux, vy = uy, vz = uz;   // the stored velocity includes F/(2 rho)
+        if (has_force) {
+            const R hi = R(0.5) / rho;
+            vx = ux + Fx * hi; vy = uy + Fy * hi; vz = uz + Fz * hi;
+        }
+        if (LES) {
+            Vel<U, R>::st(static_cast<U*>(B.u_next[0]), cell, vx);
+            Vel<U, R>::st(static_cast<U*>(B.u_next[1]), cell, vy);
+            Vel<U, R>::st(static_cast<U*>(B.u_next[2]), cell, vz);
+        }
+        const R tl = (P.cs * smag) * (P.cs * smag);
+        R tk = tl * P.vel_scale2;
+        if (!fin(tk) || tk < R(0)) tk = R(0);
+        if (tk > R(1.0e6)) tk = R(1.0e6);
+        if (P.write_fields) {
+            static_cast<R*>(B.fld[0])[cell] = rho;
+            static_cast<R*>(B.fld[1])[cell] = vx;
+            static_cast<R*>(B.fld[2])[cell] = vy;
+            static_cast<R*>(B.fld[3])[cell] = vz;
+            static_cast<R*>(B.fld[4])[cell] = tk;
+            static_cast<R*>(B.fld[5])[cell] = smag;
+        }
+        const float vmag = (float)__builtin_sqrt(usqr);
+        acc.maxv = fmaxf(acc.maxv, vmag);
+        acc.rho += (double)rho;
+        acc.ke += (double)usqr;
+        acc.tke += (double)tk;
+        acc.nan += nanc; acc.bad += badc; acc.neg += negc;
+
+        // ---------------- D3Q7 temperature: pull, then BGK with the new velocity
+        // (aeros/src/thermal_lbm.cpp:205-336)
+        if (THERM) {
+            const R* g = static_cast<const R*>(B.g_in);
+            R* go = static_cast<R*>(B.g_out);
+            const int x = (int)(cell % nx);
+            const int y = (int)((cell / nx) % ny);
+            const int z = (int)(cell / sz);
+            R gi[Q7];
+#pragma unroll
+            for (int i = 0; i < Q7; ++i) {
+                int sx = x - cx_of(i), syy = y - cy_of(i), szz = z - cz_of(i);
+                // x and z always periodic for the temperature lattice
+                if (sx < 0) sx = nx - 1; else if (sx >= nx) sx = 0;
+                if (szz < 0) szz = nz - 1; else if (szz >= nz) szz = 0;
+                const bool outy = syy < 0 || syy >= ny;
+                bool bounce = outy;
+                int wy = syy < 0 ? 0 : ny - 1;
+                if (!outy) {
+                    const int64_t src = ((int64_t)szz * ny + syy) * nx + sx;
+                    bounce = (B.mask[src] & M_SOLID) != 0;
+                    wy = syy;
+                }
+                if (bounce) {
+                    const R go_ = g[opp_of(i) * n + cell];
+                    // anti-bounce-back with the wall temperature (aeros/src/thermal_lbm.cpp:60-65, 286-293)
+                    if (P.t_fixed && (wy <= 0 || wy >= ny - 1))
+                        gi[i] = -go_ + R(2) * w7_of<R>(i) * (wy <= 0 ? P.t_hot : P.t_cold);
+                    else
+                        gi[i] = go_;
+                } else {
+                    gi[i] = g[i * n + (((int64_t)szz * ny + syy) * nx + sx)];
+                }
+            }
+            R T = R(0);
+#pragma unroll
+            for (int i = 0; i < Q7; ++i) T = i == 0 ? gi[0] : T + gi[i];
+            if (!fin(T)) T = P.t_ref;
+            T = (P.t_lo > T ? P.t_lo : T);
+            T = (P.t_hi < T ? P.t_hi : T);
+#pragma unroll
+            for (int i = 0; i < Q7; ++i) {
+                const R cu = (R)cx_of(i) * vx + (R)cy_of(i) * vy + (R)cz_of(i) * vz;
+                const R e = w7_of<R>(i) * T * (R(1) + R(4) * cu);
+                go[i * n + cell] = gi[i] - (gi[i] - e) * P.t_inv_tau;
+            }
+            static_cast<R*>(B.t_out)[cell] = T;
+            static_cast<R*>(B.b_out)[cell] = P.t_g * (T - P.t_ref);
+        }
+    }
+
+    // ---------------- fixed-order reduction and end-of-step logic
+    if (P.dry) return;
+    block_reduce(L, tid, acc);
+    if (tid == 0) {
+        B.parts[blockIdx.x] = acc;
+        __threadfence();
+        const uint32_t t = atomicAdd(&st->ticket, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (is_last) {
+        __threadfence();
+        finalize_step<R>(P, B.parts, st, L, tid);
+    }
+}
+
+// ------------------------------------------------------------ other kernels
+// Fill both lattices: buffer A gets the initial equilibria, buffer B zeros
+// (the reference's f / fNext after initLBM, aeros/src/lbm.cpp:581-582, 690-739).
+template <class R, class S>
+__global__ void k_init(S* fa, S* fb, const uint32_t* mask, const float* u3, int64_t n, R u0, R u1, R u2) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const bool sol = mask[c] & M_SOLID;
+        R a = R(0), b = R(0), d = R(0);
+        if (!sol) {
+            if (u3) { a = (R)u3[c * 3]; b = (R)u3[c * 3 + 1]; d = (R)u3[c * 3 + 2]; }
+            else { a = u0; b = u1; d = u2; }
+        }
+        R us = a * a + b * b + d * d;
+        if (us > R(0.09)) {
+            const R s = __builtin_sqrt(R(0.09) / us);
+            a *= s; b *= s; d *= s; us = a * a + b * b + d * d;
+        }
+        for (int q = 0; q < Q; ++q) {
+            Pop<S, R>::st(fa, q * n + c, feq_guarded<R>(q, R(1), a, b, d, us), w_of<R>(q));
+            Pop<S, R>::st(fb, q * n + c, R(0), w_of<R>(q));
+        }
+    }
+}
+
+// Divergence reset of the fluid cells (aeros/src/lbm.cpp:1583-1600).
+template <class R, class S, class U>
+__global__ void k_reset(S* f, const uint32_t* mask, int64_t n, StepParams<R> P, U* u0, U* u1, U* u2, R* rho,
+                        R* fux, R* fuy, R* fuz) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        if (mask[c] & M_SOLID) continue;
+        for (int q = 0; q < Q; ++q) Pop<S, R>::st(f, q * n + c, P.feq_in[q], w_of<R>(q));
+        if (u0) { Vel<U, R>::st(u0, c, P.in_u[0]); Vel<U, R>::st(u1, c, P.in_u[1]); Vel<U, R>::st(u2, c, P.in_u[2]); }
+        rho[c] = R(1); fux[c] = P.in_u[0]; fuy[c] = P.in_u[1]; fuz[c] = P.in_u[2];
+    }
+}
+
+// Per-cell link masks from the solid/ground/wall-distance bytes.
+__global__ void k_build_mask(uint32_t* mask, const uint8_t* solid, const uint8_t* ground, const uint8_t* wall, int nx,
+                             int ny, int nz, int axis, int sign, int periodic_x, int periodic_z, int conv_outlet) {
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / sz);
+        uint32_t m = 0;
+        const bool sol = solid[c] || ground[c];
+        if (sol) m |= M_SOLID;
+        const int p[3] = {x, y, z}, N[3] = {nx, ny, nz};
+        const bool inl = sign > 0 ? p[axis] == 0 : p[axis] == N[axis] - 1;
+        const bool outl = sign > 0 ? p[axis] == N[axis] - 1 : p[axis] == 0;
+        if (inl) m |= M_INLET;
+        if (outl) m |= M_OUTLET;
+        const bool shell = x == 0 || x == nx - 1 || y == 0 || y == ny - 1 || z == 0 || z == nz - 1;
+        if (shell) m |= M_SHELL;
+        if (!sol) {
+            if (inl && !periodic_x) m |= M_PULL_INLET;
+            else if (outl && conv_outlet && !periodic_x) m |= M_PULL_OUTLET;
+            for (int q = 0; q < Q; ++q) {
+                int sx = x - cx_of(q), syy = y - cy_of(q), szz = z - cz_of(q);
+                if (periodic_x) { if (sx < 0) sx += nx; else if (sx >= nx) sx -= nx; }
+                if (periodic_z) { if (szz < 0) szz += nz; else if (szz >= nz) szz -= nz; }
+                if (sx < 0 || sx >= nx || syy < 0 || syy >= ny || szz < 0 || szz >= nz) continue;
+                const int64_t s = ((int64_t)szz * ny + syy) * nx + sx;
+                if (solid[s] || ground[s]) m |= 1u << q;
+            }
+            // strain defined: interior, off the inlet/outlet planes, no solid
+            // 6-neighbour (ground neighbours count as fluid here, as in
+            // aeros/src/lbm.cpp:1016-1022)
+            if (!inl && !outl && !shell && !solid[c - 1] && !solid[c + 1] && !solid[c - sy] && !solid[c + sy] &&
+                !solid[c - sz] && !solid[c + sz])
+                m |= M_STRAIN;
+        }
+        const int wd = wall[c] <= 5 ? wall[c] : 6;
+        m |= (uint32_t)wd << M_VD_SHIFT;
+        mask[c] = m;
+    }
+}
+
+// Post-collision -> post-stream: materialise the populations the reference
+// holds after the same number of steps (its f).  Fluid cells pull exactly as
+// the step kernel does; solid cells show the untouched buffer of that parity.
+template <class R, class S>
+__global__ void k_pull_only(const S* fin_, const S* fsolid, R* out, const uint32_t* mask, StepParams<R> P) {
+    const int64_t n = P.n;
+    const int nx = P.nx, ny = P.ny, nz = P.nz;
+    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    for (int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; cell < n; cell += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t m = mask[cell];
+        if (m & M_SOLID) {
+            for (int q = 0; q < Q; ++q) out[q * n + cell] = Pop<S, R>::ld(fsolid, q * n + cell, w_of<R>(q));
+            continue;
+        }
+        if (m & M_PULL_INLET) {
+            for (int q = 0; q < Q; ++q) out[q * n + cell] = P.feq_in[q];
+            continue;
+        }
+        if (m & M_PULL_OUTLET) {
+            const int64_t ci = cell - P.out_d1, ci2 = cell - P.out_d2;
+            for (int q = 0; q < Q; ++q) {
+                const R w = w_of<R>(q);
+                const R fo = Pop<S, R>::ld(fin_, q * n + cell, w), fi = Pop<S, R>::ld(fin_, q * n + ci, w);
+                const R fi2 = Pop<S, R>::ld(fin_, q * n + ci2, w);
+                out[q * n + cell] = fo + P.conv_c * (fi - fo) + P.conv_k2 * (fi - fi2);
+            }
+            continue;
+        }
+        const int x = (int)(cell % nx), y = (int)((cell / nx) % ny), z = (int)(cell / sz);
+        for (int q = 0; q < Q; ++q) {
+            int sx = x - cx_of(q), syy = y - cy_of(q), szz = z - cz_of(q);
+            if (P.periodic_x) { if (sx < 0) sx += nx; else if (sx >= nx) sx -= nx; }
+            if (P.periodic_z) { if (szz < 0) szz += nz; else if (szz >= nz) szz -= nz; }
+            const bool out_ = sx < 0 || sx >= nx || syy < 0 || syy >= ny || szz < 0 || szz >= nz;
+            const bool bb = !out_ && ((m >> q) & 1u);
+            const int64_t src = (out_ || bb) ? cell : ((int64_t)szz * ny + syy) * nx + sx;
+            const int dq = bb ? opp_of(q) : q;
+            out[q * n + cell] = Pop<S, R>::ld(fin_, dq * n + src, w_of<R>(dq));
+        }
+        (void)sy;
+    }
+}
+
+// Expand storage to R (populations of a pre-collision state).
+template <class R, class S>
+__global__ void k_widen(const S* in, R* out, int64_t n) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
+        for (int q = 0; q < Q; ++q) out[q * n + c] = Pop<S, R>::ld(in, q * n + c, w_of<R>(q));
+}
+
+// thermalUpload / thermalResetField populations: g = g_eq(T, 0) = w T.
+template <class R>
+__global__ void k_thermal_eq(R* g, const R* T, int64_t n) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const R t = T[c];
+        for (int i = 0; i < Q7; ++i) g[i * n + c] = w7_of<R>(i) * t * (R(1) + R(4) * R(0));
+    }
+}
+
+// World velocity and pressure post-pass (aeros/src/lbm.cpp:1602-1660).
+template <class R>
+__global__ void k_post(const uint32_t* mask, const R* rho, const R* ux, const R* uy, const R* uz, R* wx, R* wy,
+                       R* wz, R* p, int64_t n, R scale, R rho0, unsigned long long* bad) {
+    unsigned long long nb = 0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        if (mask[c] & M_SOLID) { wx[c] = wy[c] = wz[c] = p[c] = R(0); continue; }
+        R vx = ux[c];
+        if (!fin(vx)) vx = R(0);
+        wx[c] = vx * scale; wy[c] = uy[c] * scale; wz[c] = uz[c] * scale;
+        R r = rho[c];
+        if (!fin(r)) r = R(1);
+        const R pl = (R(1) / R(3)) * (r - R(1));
+        R pp = pl * scale * scale * rho0;
+        if (!fin(pp)) { nb++; pp = R(0); }
+        p[c] = pp;
+    }
+    if (nb) atomicAdd(bad, nb);
+}
+
+// Sampled pressure range and rho residual (aeros/src/lbm.cpp:1662-1705):
+// cells 0, stride, 2 stride, ...; one block, fixed order.
+template <class R>
+__global__ void k_post_scalars(const R* p, const R* rho, float* prev, int64_t n, int64_t stride, int valid,
+                               float* out /* pmin, pmax, resid, nsampled */) {
+    __shared__ float smin[BLOCK], smax[BLOCK];
+    __shared__ double ss[BLOCK];
+    __shared__ int sn[BLOCK];
+    const int tid = threadIdx.x;
+    float mn = 1e30f, mx = -1e30f;
+    double acc = 0.0;
+    int cnt = 0;
+    const int64_t ns = (n + stride - 1) / stride;
+    for (int64_t k = tid; k < ns; k += BLOCK) {
+        const int64_t c = k * stride;
+        const float pv = (float)p[c];
+        if (__builtin_isfinite(pv)) { mn = fminf(mn, pv); mx = fmaxf(mx, pv); }
+        const float r = (float)rho[c];
+        if (valid && __builtin_isfinite(r)) { const float d = r - prev[k]; acc += (double)d * (double)d; cnt++; }
+        prev[k] = __builtin_isfinite(r) ? r : 1.0f;
+    }
+    smin[tid] = mn; smax[tid] = mx; ss[tid] = acc; sn[tid] = cnt;
+    for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        __syncthreads();
+        if (tid < off) {
+            smin[tid] = fminf(smin[tid], smin[tid + off]);
+            smax[tid] = fmaxf(smax[tid], smax[tid + off]);
+            ss[tid] += ss[tid + off];
+            sn[tid] += sn[tid + off];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const float a = smin[0], b = smax[0];
+        out[0] = (a <= b && __builtin_isfinite(a)) ? a : 0.0f;
+        out[1] = (a <= b && __builtin_isfinite(b)) ? b : 0.0f;
+        out[2] = (valid && sn[0] > 0) ? (float)__builtin_sqrt(ss[0] / (double)sn[0]) : 0.0f;
+        out[3] = (float)sn[0];
+    }
+}
+
+// Control-volume momentum balance (aeros/src/lbm.cpp:2183-2232): each face
+// cell's float term is formed as the reference forms it and accumulated in
+// double; per-block partials are folded in block order by the host.
+template <class R>
+__global__ void k_cv_force(const uint32_t* solid_mask, const uint8_t* solid, const R* rho, const R* ux, const R* uy,
+                           const R* uz, int nx, int ny, int nz, int x0, int x1, int y0, int y1, int z0, int z1,
+                           double* partial) {
+    __shared__ double s[3][BLOCK];
+    const int tid = threadIdx.x;
+    const int64_t nyz = (int64_t)(z1 - z0 + 1) * (y1 - y0 + 1), nzx = (int64_t)(z1 - z0 + 1) * (x1 - x0 + 1),
+                  nyx = (int64_t)(y1 - y0 + 1) * (x1 - x0 + 1);
+    const int64_t total = 2 * (nyz + nzx + nyx);
+    double fx = 0, fy = 0, fz = 0;
+    for (int64_t k = (int64_t)blockIdx.x * BLOCK + tid; k < total; k += (int64_t)gridDim.x * BLOCK) {
+        int x, y, z;
+        R nxv = 0, nyv = 0, nzv = 0;
+        int64_t r = k;
+        if (r < 2 * nyz) {
+            const int64_t pair = r / 2; const int side = (int)(r % 2);
+            z = z0 + (int)(pair / (y1 - y0 + 1)); y = y0 + (int)(pair % (y1 - y0 + 1));
+            x = side ? x1 : x0; nxv = side ? R(1) : R(-1);
+        } else if ((r -= 2 * nyz) < 2 * nzx) {
+            const int64_t pair = r / 2; const int side = (int)(r % 2);
+            z = z0 + (int)(pair / (x1 - x0 + 1)); x = x0 + (int)(pair % (x1 - x0 + 1));
+            y = side ? y1 : y0; nyv = side ? R(1) : R(-1);
+        } else {
+            r -= 2 * nzx;
+            const int64_t pair = r / 2; const int side = (int)(r % 2);
+            y = y0 + (int)(pair / (x1 - x0 + 1)); x = x0 + (int)(pair % (x1 - x0 + 1));
+            z = side ? z1 : z0; nzv = side ? R(1) : R(-1);
+        }
+        const int64_t c = ((int64_t)z * ny + y) * nx + x;
+        if (solid[c]) continue;
+        const R rr = rho[c], vx = ux[c], vy = uy[c], vz = uz[c];
+        const R pp = (R(1) / R(3)) * (rr - R(1));
+        const R un = vx * nxv + vy * nyv + vz * nzv;
+        fx += (double)(pp * nxv + rr * vx * un);
+        fy += (double)(pp * nyv + rr * vy * un);
+        fz += (double)(pp * nzv + rr * vz * un);
+    }
+    (void)solid_mask;
+    s[0][tid] = fx; s[1][tid] = fy; s[2][tid] = fz;
+    for (int k = 0; k < 3; ++k) lds_tree_sum(s[k], tid);
+    if (tid == 0) { partial[blockIdx.x * 3] = s[0][0]; partial[blockIdx.x * 3 + 1] = s[1][0]; partial[blockIdx.x * 3 + 2] = s[2][0]; }
+}
+
+// Exact checksum of the rho field: in the float modes every rho lies in
+// [0.5, 1.5] and is an integer multiple of 2^-24, so the sum is accumulated
+// exactly in 64-bit integers (order independent, equal to any exact or
+// sequential double sum).  The u-sum and the FP64 mode use a fixed-order
+// double tree.
+template <class R>
+__global__ void k_checksums(const R* rho, const R* ux, const R* uy, const R* uz, int64_t n, double* part,
+                            unsigned long long* exact, int exact_ok) {
+    __shared__ double s[2][BLOCK];
+    const int tid = threadIdx.x;
+    double a = 0, b = 0;
+    unsigned long long e = 0;
+    for (int64_t c = (int64_t)blockIdx.x * BLOCK + tid; c < n; c += (int64_t)gridDim.x * BLOCK) {
+        a += (double)rho[c];
+        b += (double)(ux[c] + uy[c] + uz[c]);
+        if (exact_ok) e += (unsigned long long)((double)rho[c] * 16777216.0);
+    }
+    s[0][tid] = a; s[1][tid] = b;
+    lds_tree_sum(s[0], tid);
+    lds_tree_sum(s[1], tid);
+    if (tid == 0) { part[blockIdx.x * 2] = s[0][0]; part[blockIdx.x * 2 + 1] = s[1][0]; }
+    if (exact_ok && e) atomicAdd(exact, e);
+}
+
+// Projected area: number of (a, b) index pairs across the flow axis with at
+// least one solid cell along it (aeros/src/lbm.cpp:2236-2256).
+__global__ void k_proj_area(const uint8_t* solid, int nx, int ny, int nz, int axis, unsigned long long* count) {
+    const int na = axis == 0 ? ny : nx, nb = axis == 2 ? ny : nz, nl = axis == 0 ? nx : (axis == 1 ? ny : nz);
+    const int64_t np = (int64_t)na * nb;
+    unsigned long long cnt = 0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < np; k += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(k % na), b = (int)(k / na);
+        bool hit = false;
+        for (int l = 0; l < nl && !hit; ++l) {
+            int x, y, z;
+            if (axis == 0) { x = l; y = a; z = b; }
+            else if (axis == 1) { x = a; y = l; z = b; }
+            else { x = a; y = b; z = l; }
+            hit = solid[((int64_t)z * ny + y) * nx + x] != 0;
+        }
+        cnt += hit;
+    }
+    if (cnt) atomicAdd(count, cnt);
+}
+
+// lbmEquilibriumMoments (aeros/src/lbm.cpp:249-266) on the device.
+__global__ void k_eq_moments(float rho, float ux, float uy, float uz, float* out, int* ok) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (!__builtin_isfinite(rho) || !__builtin_isfinite(ux) || !__builtin_isfinite(uy) || !__builtin_isfinite(uz)) {
+        *ok = 0;
+        return;
+    }
+    const float us = ux * ux + uy * uy + uz * uz;
+    float m = 0, px = 0, py = 0, pz = 0;
+    for (int q = 0; q < Q; ++q) {
+        const float fe = feq_guarded<float>(q, rho, ux, uy, uz, us);
+        if (!__builtin_isfinite(fe)) { *ok = 0; return; }
+        m += fe; px += fe * (float)cx_of(q); py += fe * (float)cy_of(q); pz += fe * (float)cz_of(q);
+    }
+    out[0] = m; out[1] = px; out[2] = py; out[3] = pz;
+    *ok = 1;
+}
+
+// Mean temperature over the fluid rows and the Nusselt sum at the first
+// fluid row (aeros/src/thermal_lbm.cpp:67-80, 338-351); block partials.
+template <class R>
+__global__ void k_thermal_diag(const R* T, const uint32_t* mask, int nx, int ny, int nz, int ylo, int yhi, R hot,
+                               double* part) {
+    __shared__ double s[3][BLOCK];
+    const int tid = threadIdx.x;
+    const int64_t rows = (int64_t)(yhi - ylo + 1) * nx;
+    const int64_t m = rows * nz;
+    double st = 0, sq = 0, sn = 0;
+    for (int64_t k = (int64_t)blockIdx.x * BLOCK + tid; k < m; k += (int64_t)gridDim.x * BLOCK) {
+        const int z = (int)(k / rows), r = (int)(k % rows), y = ylo + r / nx, x = r % nx;
+        const int64_t c = ((int64_t)z * ny + y) * nx + x;
+        if (mask[c] & M_SOLID) continue;
+        st += (double)T[c];
+        if (y == ylo) { sq += (double)(hot - T[c]) / 0.5; sn += 1.0; }
+    }
+    s[0][tid] = st; s[1][tid] = sq; s[2][tid] = sn;
+    for (int k = 0; k < 3; ++k) lds_tree_sum(s[k], tid);
+    if (tid == 0) for (int k = 0; k < 3; ++k) part[blockIdx.x * 3 + k] = s[k][0];
+}
+
+__global__ void k_probe(int* d) { d[threadIdx.x] += 1; }
+
+// Fill a typed buffer with a value (graph-capturable, unlike hipMemset).
+template <class T> __global__ void k_fill(T* p, int64_t n, T v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ------------------------------------------------------------ launchers
+template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
+static hipError_t launch_step_t(const StepParams<R>& P, const StepBuffers& B, int grid, hipStream_t s) {
+    hipLaunchKernelGGL((k_step<R, S, U, LES, THERM, FORCE>), dim3(grid), dim3(BLOCK), 0, s, P, B);
+    return hipGetLastError();
+}
+
+template <class R, class S, class U>
+hipError_t launch_step(const StepParams<R>& P, const StepBuffers& B, int grid, hipStream_t s) {
+    const bool force = P.fx != R(0) || P.fy != R(0) || P.fz != R(0);
+    if (P.les) {
+        if (P.thermal) return force ? launch_step_t<R, S, U, true, true, true>(P, B, grid, s)
+                                    : launch_step_t<R, S, U, true, true, false>(P, B, grid, s);
+        return force ? launch_step_t<R, S, U, true, false, true>(P, B, grid, s)
+                     : launch_step_t<R, S, U, true, false, false>(P, B, grid, s);
+    }
+    if (P.thermal) return force ? launch_step_t<R, S, U, false, true, true>(P, B, grid, s)
+                                : launch_step_t<R, S, U, false, true, false>(P, B, grid, s);
+    return force ? launch_step_t<R, S, U, false, false, true>(P, B, grid, s)
+                 : launch_step_t<R, S, U, false, false, false>(P, B, grid, s);
+}
+
+}  // namespace aeros

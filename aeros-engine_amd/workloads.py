@@ -1,0 +1,219 @@
+"""Synthetic cases: the reference's own test geometries and the measured workloads.
+
+Each builder returns a dict of ``aeros_lbm_config`` overrides (voxel grid,
+box, flow and solver parameters) that both the HIP solver and the CPU oracle
+accept.  Geometry is evaluated in float32 exactly as the reference's C++ test
+generators do, so the voxel masks are bit-identical to theirs.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+f32 = np.float32
+
+
+def _centres(n: int, lo: float, cs: np.float32) -> np.ndarray:
+    # lo + (i + 0.5f) * cs, all float32
+    return (f32(lo) + (np.arange(n, dtype=f32) + f32(0.5)) * cs).astype(f32)
+
+
+def sphere_voxels(n: int, lo: float, hi: float, radius: float) -> np.ndarray:
+    """(n,n,n) voxel grid of a sphere of radius ``radius`` centred at 0 in the cube [lo,hi]^3.
+
+    Same rule as aeros/tests/bench_lbm.cpp:45-60 and
+    aeros/tests/validation_headless.cpp:57-81: solid iff |p| - R <= 0 at the
+    voxel centre, evaluated in float."""
+    cs = f32(f32(hi - lo) / f32(n))
+    c = _centres(n, lo, cs)
+    pz, py, px = np.meshgrid(c, c, c, indexing="ij")
+    d = np.sqrt(px * px + py * py + pz * pz).astype(f32) - f32(radius)
+    return (d <= f32(0)).astype(np.int32)
+
+
+def cylinder_voxels(n: int, lo: float, hi: float, radius: float) -> np.ndarray:
+    """Cylinder along x (aeros/tests/validation_headless.cpp:68-72)."""
+    cs = f32(f32(hi - lo) / f32(n))
+    c = _centres(n, lo, cs)
+    pz, py, _px = np.meshgrid(c, c, c, indexing="ij")
+    d = np.sqrt(py * py + pz * pz).astype(f32) - f32(radius)
+    return (d <= f32(0)).astype(np.int32)
+
+
+def bench_sphere(n: int = 32, les: bool = True, thermal: bool = False) -> dict:
+    """The reference bench (aeros/tests/bench_lbm.cpp:35-79): sphere R=0.35 in [-1,1]^3 at n^3,
+    padding 0.25, tau 0.6, U0 0.08, LES on, convective outlet, flow 20 m/s.
+    n=32 gives the 80x72x72 domain of the published checksum."""
+    vox = sphere_voxels(n, -1.0, 1.0, 0.35)
+    return dict(
+        voxels=vox, vox_min=(-1.0, -1.0, -1.0), vox_max=(1.0, 1.0, 1.0), voxel_resolution=n,
+        max_dim=0.7, flow_speed=20.0, flow_azimuth_deg=0.0, flow_elevation_deg=0.0,
+        U0=0.08, tau=0.6, use_les=int(les), use_ground=0, inlet_turbulence=0.0,
+        domain_padding=0.25, auto_viscosity=0, convective_outlet=1, periodic_x=0,
+        body_force=(0.0, 0.0, 0.0), thermal=int(thermal), rayleigh=2500.0, prandtl=1.0,
+    )
+
+
+def lbm_suite_sphere(les: bool = True) -> dict:
+    """makeSphereCase(0.7) of aeros/tests/lbm_physics_headless.cpp:48-63 with the
+    suite's parameters (tau 0.55, U0 0.08, Cs 0.14, padding 0.25)."""
+    vox = sphere_voxels(24, -3.0, 3.0, 0.7)
+    return dict(
+        voxels=vox, vox_min=(-3.0, -3.0, -3.0), vox_max=(3.0, 3.0, 3.0), voxel_resolution=24,
+        max_dim=1.4, flow_speed=20.0, tau=0.55, auto_viscosity=0, U0=0.08, smagorinsky_c=0.14,
+        use_ground=0, domain_padding=0.25, use_les=int(les),
+    )
+
+
+def channel(ny: int = 34, tau: float = 0.65, force: float = 7.8e-6, trt: bool = False) -> dict:
+    """Poiseuille channel 8 x ny x 8 (aeros/tests/physics_ref_headless.cpp:65-118):
+    solid rows y=0 and y=ny-1, periodic x and z, Guo body force along x."""
+    nx = nz = 8
+    vox = np.zeros((nz, ny, nx), dtype=np.int32)
+    vox[:, 0, :] = 1
+    vox[:, ny - 1, :] = 1
+    return dict(
+        voxels=vox, vox_min=(-1.0, -1.0, -1.0), vox_max=(1.0, 1.0, 1.0), voxel_resolution=nx,
+        max_dim=2.0, flow_speed=1.0, U0=0.0, tau=float(tau), use_les=0, use_ground=0,
+        inlet_turbulence=0.0, domain_padding=0.0, auto_viscosity=0, convective_outlet=0,
+        use_trt=int(trt), periodic_x=1, periodic_z=1, body_force=(float(force), 0.0, 0.0),
+    )
+
+
+def thermal_box(nx=32, ny=16, nz=8, walls_y=False, u0=0.0, tau=0.65, periodic_z=True,
+                rayleigh=0.0, prandtl=1.0, fixed_wall_temps=False, perturbation=0.0,
+                hot=1.0, cold=0.0) -> dict:
+    """setupBox of aeros/tests/thermal_headless.cpp:54-99 plus the thermal parameters."""
+    vox = np.zeros((nz, ny, nx), dtype=np.int32)
+    if walls_y:
+        vox[:, 0, :] = 1
+        vox[:, ny - 1, :] = 1
+    return dict(
+        voxels=vox, vox_min=(-1.0, -1.0, -1.0), vox_max=(1.0, 1.0, 1.0), voxel_resolution=nx,
+        max_dim=2.0, flow_speed=1.0, U0=float(u0), tau=float(tau), use_les=0, use_ground=0,
+        inlet_turbulence=0.0, domain_padding=0.0, auto_viscosity=0, convective_outlet=0,
+        periodic_x=1, periodic_z=int(periodic_z), body_force=(0.0, 0.0, 0.0),
+        thermal=1, rayleigh=float(rayleigh), prandtl=float(prandtl),
+        fixed_wall_temps=int(fixed_wall_temps), hot_temp=float(hot), cold_temp=float(cold),
+        perturbation=float(perturbation),
+    )
+
+
+def validate_sphere(res=48, diameter_cells=20, re_lat=100.0, pad=0.15, u0=0.08) -> dict:
+    """Sphere of aeros/tests/validation_headless.cpp:99-135 (BGK, no LES, convective outlet):
+    res 48, D 20 gives the 92x84x84 domain of the published Cd 1.128."""
+    radius = f32(3.0) * f32(diameter_cells) / f32(res)
+    vox = sphere_voxels(res, -3.0, 3.0, float(radius))
+    D = f32(diameter_cells)
+    tau = f32(0.5) + f32(3.0) * f32(u0) * D / f32(re_lat)
+    tau = max(f32(0.505), min(tau, f32(1.4)))
+    return dict(
+        voxels=vox, vox_min=(-3.0, -3.0, -3.0), vox_max=(3.0, 3.0, 3.0), voxel_resolution=res,
+        max_dim=float(f32(2.0) * radius), flow_speed=20.0, U0=float(u0), tau=float(tau), use_les=0,
+        use_ground=0, inlet_turbulence=0.0, domain_padding=float(pad), auto_viscosity=0,
+        convective_outlet=1, periodic_x=0,
+    )
+
+
+# --------------------------------------------------------------- measured workloads
+
+def sphere_in_box(nx, ny, nz, centre, diameter) -> np.ndarray:
+    """Sphere by the cell-centre test |x - c| <= D/2 in lattice units (the bench rule)."""
+    z, y, x = np.meshgrid(np.arange(nz, dtype=f32) + f32(0.5), np.arange(ny, dtype=f32) + f32(0.5),
+                          np.arange(nx, dtype=f32) + f32(0.5), indexing="ij")
+    cx, cy, cz = (f32(v) for v in centre)
+    dx, dy, dz = x - cx, y - cy, z - cz
+    d = np.sqrt(dx * dx + dy * dy + dz * dz).astype(f32)
+    return (d <= f32(diameter) * f32(0.5)).astype(np.int32)
+
+
+def lattice_box(nx, ny, nz, cell=1.0):
+    """Voxel box whose cells coincide with the lattice (padding 0): world = lattice units."""
+    return dict(vox_min=(0.0, 0.0, 0.0), vox_max=(nx * cell, ny * cell, nz * cell), domain_padding=0.0)
+
+
+def c1_sphere_fp64() -> dict:
+    """96x48x48, sphere D=12 at (24,24,24), U0 0.05 along +x, BGK, tau = 0.5 + 3 U0 D / Re (Re 100)."""
+    nx, ny, nz = 96, 48, 48
+    vox = sphere_in_box(nx, ny, nz, (24, 24, 24), 12)
+    tau = float(f32(0.5) + f32(3.0) * f32(0.05) * f32(12.0) / f32(100.0))
+    return dict(voxels=vox, **lattice_box(nx, ny, nz), voxel_resolution=nx, max_dim=12.0,
+                flow_speed=20.0, U0=0.05, tau=tau, use_les=0, use_trt=0, auto_viscosity=0,
+                inlet_turbulence=0.0, convective_outlet=0, precision="fp64")
+
+
+def ellipsoid_mesh(semi=(30.0, 12.0, 12.0), centre=(0.0, 0.0, 0.0), n_u=316, n_v=316,
+                   jitter=0.05, seed=12345) -> np.ndarray:
+    """Closed UV-parametrised ellipsoid, (T, 9) float32 triangles, T = 2 n_u (n_v - 1).
+
+    n_u = n_v = 316 gives 199,080 triangles.  Every vertex (poles included) gets
+    a jitter drawn from U(-jitter, jitter) per axis by a seeded PCG64 generator;
+    shared vertices move together so the surface stays closed."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    a, b, c = semi
+    th = np.linspace(0.0, np.pi, n_v + 1)[1:-1]          # interior rings
+    ph = np.linspace(0.0, 2.0 * np.pi, n_u, endpoint=False)
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    ring = np.stack([a * np.cos(T), b * np.sin(T) * np.cos(P), c * np.sin(T) * np.sin(P)], axis=-1)
+    north = np.array([a, 0.0, 0.0])
+    south = np.array([-a, 0.0, 0.0])
+    verts = np.concatenate([north[None], ring.reshape(-1, 3), south[None]], axis=0)
+    verts = verts + rng.uniform(-jitter, jitter, size=verts.shape)
+    verts = verts + np.asarray(centre)[None, :]
+    nr = n_v - 1
+    idx = lambda i, j: 1 + i * n_u + (j % n_u)  # noqa: E731
+    tris = []
+    for j in range(n_u):
+        tris.append((0, idx(0, j), idx(0, j + 1)))
+    for i in range(nr - 1):
+        for j in range(n_u):
+            a0, a1, b0, b1 = idx(i, j), idx(i, j + 1), idx(i + 1, j), idx(i + 1, j + 1)
+            tris.append((a0, b0, b1))
+            tris.append((a0, b1, a1))
+    last = len(verts) - 1
+    for j in range(n_u):
+        tris.append((last, idx(nr - 1, j + 1), idx(nr - 1, j)))
+    tri = np.asarray(tris, dtype=np.int64)
+    return verts[tri].reshape(-1, 9).astype(f32)
+
+
+def c2_config(voxelize_fn, nx=256, ny=128, nz=104, scale=1.0, seed=12345, re=None,
+              precision="fp32", with_mask=True) -> dict:
+    """256x128x104, TRT (magic 3/16) + Smagorinsky Cs 0.1, U0 0.08, no body force, an ellipsoid
+    STL (semi-axes 30/12/12 cells, ~200k triangles, vertex jitter 0.05 cells, seeded) voxelized
+    by the BVH three-ray vote at cell centres.  tau is stated: 0.5 + 3 U0 D / Re with D = 24 (the
+    minor-axis diameter) and Re = 300 -> 0.5192; scale=2 gives the 512x256x256 case with Re 1000."""
+    nx, ny, nz = int(nx * scale), int(ny * scale), int(nz * scale)
+    semi = (30.0 * scale, 12.0 * scale, 12.0 * scale)
+    centre = (nx * 0.3, ny * 0.5, nz * 0.5)
+    cfg = dict(**lattice_box(nx, ny, nz), voxel_resolution=nx, max_dim=float(2 * semi[0]),
+               flow_speed=20.0, U0=0.08, use_les=1, smagorinsky_c=0.1, use_trt=1,
+               auto_viscosity=0, inlet_turbulence=0.0, convective_outlet=1, body_force=(0.0, 0.0, 0.0),
+               precision=precision, cell_cap=1 << 31)
+    D = 24.0 * scale
+    Re = re if re is not None else (300.0 if scale == 1.0 else 1000.0)
+    cfg["tau"] = float(f32(0.5) + f32(3.0) * f32(0.08) * f32(D) / f32(Re))
+    if with_mask:
+        tris = ellipsoid_mesh(semi=semi, centre=centre, seed=seed, jitter=0.05)
+        cfg["voxels"] = voxelize_fn(tris, (nx, ny, nz), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    else:
+        cfg["voxels"] = np.zeros((nz, ny, nx), dtype=np.int32)
+    return cfg
+
+
+def c3_rayleigh_benard(nx=128, ny=64, nz=128, seed=2024, rayleigh=2500.0, prandtl=0.71, tau=0.6):
+    """Coupled D3Q19 + D3Q7 Rayleigh-Benard.  The reference's gravity and walls are along y, so
+    the 128x128x64 case runs as x=128, y=64 (vertical, solid rows y=0 and y=ny-1 hot/cold),
+    z=128, periodic in x and z.  Returns (config overrides, initial T field): T linear between
+    the walls plus a seeded U(-1e-3, 1e-3) perturbation."""
+    cfg = thermal_box(nx=nx, ny=ny, nz=nz, walls_y=True, u0=0.0, tau=tau, periodic_z=True,
+                      rayleigh=rayleigh, prandtl=prandtl, fixed_wall_temps=True, perturbation=0.0)
+    cfg["cell_cap"] = 1 << 31
+    rng = np.random.Generator(np.random.PCG64(seed))
+    y = np.arange(ny, dtype=np.float64)
+    s = np.clip((y - 1.0) / max(1.0, ny - 3.0), 0.0, 1.0)
+    lin = 1.0 - s
+    T = np.broadcast_to(lin[None, :, None], (nz, ny, nx)).copy()
+    T += rng.uniform(-1e-3, 1e-3, size=T.shape)
+    T[:, 0, :] = 1.0
+    T[:, ny - 1, :] = 0.0
+    return cfg, T.astype(f32)

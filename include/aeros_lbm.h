@@ -1,0 +1,322 @@
+/*
+ * aeros_lbm.h — C ABI of the MI355X (gfx950) lattice Boltzmann solver that
+ * replaces the CPU hot path of Aeros Engine 1.38.1.
+ *
+ * Every entry point is plain C: opaque handles, plain pointers and sizes, an
+ * int status (0 = success, negative = error; the message is kept per thread
+ * and fetched with aeros_lbm_last_error).  Nothing throws across the ABI.
+ * Caller buffers are borrowed only for the duration of a call.  Calls on one
+ * handle are serialised on that handle's HIP stream; different handles are
+ * independent.  Each entry point cites the reference interface it replaces
+ * (paths relative to the reference repository root).
+ *
+ * Lattice conventions shared with the reference (they are an interface, not
+ * an implementation choice):
+ *   - cells are x-fastest, cell = (z*Ny + y)*Nx + x   (aeros/src/lbm.cpp:125)
+ *   - D3Q19 direction order: 0 rest; 1..6 = +x,-x,+y,-y,+z,-z;
+ *     7..10 = (+x+y),(-x-y),(+x-y),(-x+y); 11..14 = (+x+z),(-x-z),(+x-z),(-x+z);
+ *     15..18 = (+y+z),(-y-z),(+y-z),(-y+z)   (aeros/src/lbm.cpp:92-109)
+ *   - D3Q7 thermal directions are the first seven of the above
+ *     (aeros/src/thermal_lbm.cpp:31-41).
+ */
+#ifndef AEROS_LBM_H
+#define AEROS_LBM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AEROS_LBM_ABI_VERSION 1
+
+/* status codes */
+#define AEROS_OK               0
+#define AEROS_ERR_ARG         -1   /* bad argument (null pointer, size mismatch, ...) */
+#define AEROS_ERR_NOMEM       -2   /* device or host allocation failed              */
+#define AEROS_ERR_HIP         -3   /* HIP runtime error                             */
+#define AEROS_ERR_STATE       -4   /* handle not in a usable state                  */
+#define AEROS_ERR_TOO_LARGE   -5   /* domain exceeds the configured cell cap        */
+#define AEROS_ERR_UNSUPPORTED -6   /* option combination not built                  */
+#define AEROS_ERR_NO_DEVICE   -7   /* no usable gfx950 device                       */
+
+/* population storage precision */
+#define AEROS_FP32 0   /* float populations, float collision (the reference's arithmetic) */
+#define AEROS_FP64 1   /* double populations, double collision                           */
+#define AEROS_FP16 2   /* IEEE half deviations f_i - w_i, float collision                */
+#define AEROS_BF16 3   /* bfloat16 deviations f_i - w_i, float collision                 */
+
+/* field identifiers for aeros_lbm_read_field */
+#define AEROS_FIELD_RHO        0   /* lbmRho        (aeros/src/lbm.h:132)          */
+#define AEROS_FIELD_UX         1   /* lbmUx, lattice units, includes + F/(2 rho)   */
+#define AEROS_FIELD_UY         2
+#define AEROS_FIELD_UZ         3
+#define AEROS_FIELD_UX_WORLD   4   /* lbmUxWorld, m/s (post-pass)                   */
+#define AEROS_FIELD_UY_WORLD   5
+#define AEROS_FIELD_UZ_WORLD   6
+#define AEROS_FIELD_PRESSURE   7   /* lbmPressure, Pa (post-pass)                   */
+#define AEROS_FIELD_TKE        8   /* lbmTKEField, m^2/s^2                          */
+#define AEROS_FIELD_STRAIN     9   /* lbmStrainMag as the solver stores it (|S| LB) */
+#define AEROS_FIELD_SOLID     10   /* lbmIsSolid as 0/1                              */
+#define AEROS_FIELD_GROUND    11   /* lbmIsGround as 0/1                             */
+#define AEROS_FIELD_WALLDIST  12   /* u8 wall distance (0..255) as float            */
+#define AEROS_FIELD_T         13   /* thermalT      (aeros/src/thermal_lbm.h:46)     */
+#define AEROS_FIELD_BUOYANCY  14   /* thermalBuoyancyY                              */
+#define AEROS_FIELD_COUNT     15
+
+/* bit masks for aeros_lbm_read */
+#define AEROS_READ_RHO (1u << 0)
+#define AEROS_READ_UX  (1u << 1)
+#define AEROS_READ_UY  (1u << 2)
+#define AEROS_READ_UZ  (1u << 3)
+#define AEROS_READ_T   (1u << 4)
+
+typedef struct aeros_lbm aeros_lbm;
+
+/*
+ * Everything initLBM reads (aeros/src/lbm.cpp:405-886): the set-mask producer's
+ * voxel grid and box (g_voxelData, g_vox*; aeros/src/globals.h:150-161), the
+ * flow and atmosphere globals, LBMParams (aeros/src/lbm.h:11-75) and
+ * ThermalParams (aeros/src/thermal_lbm.h:33-41), plus the MI355X-side storage
+ * precision and cell cap.  aeros_lbm_config_defaults fills the reference's
+ * defaults.
+ */
+typedef struct aeros_lbm_config {
+    int32_t struct_size;          /* = sizeof(aeros_lbm_config) */
+    /* voxel grid (1 = solid), resampled by cell-centre world coordinates */
+    const int32_t* voxels;        /* may be NULL: empty domain of voxel_resolution^3 */
+    int32_t vox_n[3];
+    float   vox_min[3];
+    float   vox_max[3];
+    int32_t voxel_resolution;
+    /* flow / atmosphere globals */
+    float   flow_speed;           /* m/s                 */
+    float   flow_azimuth_deg;
+    float   flow_elevation_deg;
+    float   air_density;          /* kg/m^3              */
+    float   air_temperature;      /* K                   */
+    float   max_dim;              /* model size, world   */
+    float   frontal_area;         /* computeLBMRefArea() result (world units^2) for the
+                                     blockage ratio; 0 = unknown, blockage reported as 0 */
+    int32_t ground_effect;        /* aeroGroundEffect    */
+    float   ground_effect_height; /* aeroGroundHeight    */
+    /* LBMParams */
+    float   tau;
+    float   U0;
+    float   smagorinsky_c;
+    float   convergence_threshold;
+    float   domain_padding;
+    float   inlet_turbulence;
+    float   ground_height;
+    float   body_force[3];
+    int32_t use_les;
+    int32_t use_trt;
+    int32_t convective_outlet;
+    int32_t periodic_x;
+    int32_t periodic_z;
+    int32_t auto_viscosity;
+    int32_t use_ground;
+    /* ThermalParams */
+    int32_t thermal;
+    float   rayleigh;
+    float   prandtl;
+    int32_t fixed_wall_temps;
+    float   hot_temp;
+    float   cold_temp;
+    float   perturbation;
+    /* MI355X side */
+    int32_t precision;            /* AEROS_FP32 / FP64 / FP16 / BF16                       */
+    int64_t cell_cap;             /* 0 = the reference's caps (8 Mi padded, 20 Mi hard);
+                                     >0 = explicit cap on Nx*Ny*Nz                          */
+    int32_t device;               /* HIP device ordinal                                    */
+} aeros_lbm_config;
+
+/* Scalars the reference keeps in globals (aeros/src/lbm.h:77-125,
+ * aeros/src/thermal_lbm.h:46-58).  Values are those after the last step. */
+typedef struct aeros_lbm_scalars {
+    int32_t step;                 /* lbmCurrentStep        */
+    int32_t converged;            /* lbmConverged          */
+    int32_t resets;               /* divergence resets so far */
+    int32_t nx, ny, nz;
+    float   avg_rho;              /* lbmAvgRho             */
+    float   avg_ke;               /* lbmAvgKineticEnergy   */
+    float   max_u_lb;             /* lbmMaxVelocityLB      */
+    float   max_u_world;          /* lbmMaxVelocityWorld   */
+    float   tke;                  /* lbmTKE                */
+    float   convergence;          /* lbmConvergence        */
+    int64_t neg_pops;             /* lbmNegativePops       */
+    int64_t bad_pops;             /* lbmBadPops            */
+    int64_t rho_out_of_range;     /* lbmRhoOutOfRange      */
+    int64_t rho_non_finite;       /* lbmRhoNonFinite       */
+    int64_t u_limited;            /* lbmULimited           */
+    int64_t pressure_non_finite;  /* lbmPressureNonFinite  */
+    float   tau;                  /* lbmParams.tau after init */
+    float   viscosity;            /* lbmParams.viscosity   */
+    float   U0;                   /* lbmParams.U0 after init */
+    float   reynolds_phys;        /* lbmReynolds           */
+    float   reynolds_lb;          /* lbmReynoldsLB         */
+    float   blockage;             /* lbmBlockageRatio      */
+    float   pressure_min;         /* lbmPressureMin        */
+    float   pressure_max;         /* lbmPressureMax        */
+    float   residual_rho;         /* lbmResidualRho        */
+    float   residual_u;           /* lbmResidualU          */
+    float   time_ms;              /* lbmTimeMs (host wall time of the last step call) */
+    int64_t fluid_cells;
+    /* thermal */
+    float   nusselt;              /* thermalNusselt        */
+    float   thermal_mean_t;       /* thermalMeanT          */
+    float   thermal_alpha;        /* thermalAlpha          */
+    float   thermal_gravity;      /* thermalGravityLb      */
+    float   thermal_tau;          /* tau_T                 */
+    float   thermal_rayleigh;     /* thermalRayleigh       */
+    float   thermal_prandtl;      /* thermalPrandtl        */
+    int32_t thermal_ready;        /* thermalReady          */
+    int32_t thermal_ylo, thermal_yhi;
+} aeros_lbm_scalars;
+
+/* Backend description, the analogue of CudaBackendInfo (aeros/src/cuda_api.h:29-46). */
+typedef struct aeros_backend_info {
+    int32_t compiled_with_hip;
+    int32_t device_count;
+    int32_t usable;               /* 1 = a test kernel ran on the device */
+    int32_t gfx_major, gfx_minor, gfx_stepping;
+    int32_t runtime_version;
+    int32_t driver_version;
+    int64_t total_memory_mb;
+    int32_t compute_units;
+    int32_t last_error;
+    char    device_name[128];
+    char    arch_name[64];
+    char    status[512];
+} aeros_backend_info;
+
+/* ---------------------------------------------------------------- setup */
+
+/* Reference defaults of LBMParams/ThermalParams/globals (aeros/src/lbm.h:11-75,
+ * aeros/src/thermal_lbm.h:33-41, aeros/src/globals.cpp:59-145). */
+void aeros_lbm_config_defaults(aeros_lbm_config* cfg);
+
+/* initLBM (aeros/src/lbm.h:156, aeros/src/lbm.cpp:405-886): builds the domain
+ * from the voxel box plus padding, resamples the mask, computes the wall
+ * distance, initialises the lattice to equilibrium, picks tau and (if asked)
+ * initialises the thermal lattice.  Returns NULL on failure with *err set. */
+aeros_lbm* aeros_lbm_create(const aeros_lbm_config* cfg, int* err);
+
+/* shutdownLBM (aeros/src/lbm.h:158). */
+int aeros_lbm_destroy(aeros_lbm* h);
+
+/* resetLBM (aeros/src/lbm.h:157): shutdown + init with the handle's config. */
+int aeros_lbm_reset(aeros_lbm* h);
+
+/* Domain geometry (lbmNx/Ny/Nz, lbmMin*, lbmCellSize*; aeros/src/lbm.h:127-130). */
+int aeros_lbm_dims(const aeros_lbm* h, int32_t dims[3], float box_min[3], float cell_size[3]);
+
+/* Set-mask at lattice resolution (the voxel -> lattice transfer of
+ * aeros/src/lbm.cpp:596-622 already done by the caller): 1 = solid.  The
+ * wall distance is recomputed and the lattice re-initialised to equilibrium,
+ * as initLBM does when the voxel grid changes (aeros/src/lbm.cpp:1736-1760).
+ * The host buffer is copied during the call. */
+int aeros_lbm_set_mask(aeros_lbm* h, const uint8_t* solid, int32_t nx, int32_t ny, int32_t nz);
+
+/* ---------------------------------------------------------------- step */
+
+/* stepLBMCPU(steps) (aeros/src/lbm.h:160, aeros/src/lbm.cpp:907-1709),
+ * including the thermal step (aeros/src/thermal_lbm.cpp:199-355), the
+ * divergence reset and the end-of-call post-pass. */
+int aeros_lbm_step(aeros_lbm* h, int32_t steps);
+
+/* updateLBM(dt, forceSteps) (aeros/src/lbm.h:164, aeros/src/lbm.cpp:1734-1779):
+ * forceSteps >= 0 is clamped to 64; -1 uses stepsPerFrame with the frame-time
+ * rule.  steps_per_frame is LBMParams::stepsPerFrame. */
+int aeros_lbm_update(aeros_lbm* h, float dt, int32_t force_steps, int32_t steps_per_frame);
+
+/* ---------------------------------------------------------------- readback */
+
+/* Synchronous device->host copy of one field into a caller buffer of
+ * Nx*Ny*Nz floats (globals of aeros/src/lbm.h:132-154). */
+int aeros_lbm_read_field(aeros_lbm* h, int32_t field, float* out);
+/* Same, in double (exact for AEROS_FP64 handles, widened otherwise). */
+int aeros_lbm_read_field_f64(aeros_lbm* h, int32_t field, double* out);
+/* The visualizer's bundle: any subset of rho, u, T (null pointers skipped). */
+int aeros_lbm_read(aeros_lbm* h, uint32_t fields, float* rho, float* ux, float* uy, float* uz, float* T);
+/* Scalars (aeros/src/lbm.h:77-125, aeros/src/thermal_lbm.h:46-58). */
+int aeros_lbm_get_scalars(aeros_lbm* h, aeros_lbm_scalars* out);
+
+/* Raw population readback, reference layout f[cell*19+q] of the state the
+ * reference holds after the same number of steps (post-stream), as double. */
+int aeros_lbm_read_populations(aeros_lbm* h, double* out);
+
+/* ---------------------------------------------------------------- forces */
+
+/* lbmControlVolumeForce (aeros/src/lbm.h:200, aeros/src/lbm.cpp:2183-2232):
+ * momentum balance over a box; out = force on the body before the float cast. */
+int aeros_lbm_cv_force(aeros_lbm* h, float half_x, float half_y, float half_z, double out[3]);
+/* lbmProjectedAreaCells (aeros/src/lbm.h:202, aeros/src/lbm.cpp:2236-2256). */
+int aeros_lbm_projected_area(aeros_lbm* h, int32_t axis, float* out);
+/* Deterministic FP64 sums over all cells of rho and of ux+uy+uz, the bench's
+ * checksum (aeros/tests/bench_lbm.cpp:103-107). */
+int aeros_lbm_checksums(aeros_lbm* h, double* rho_sum, double* u_sum);
+
+/* ---------------------------------------------------------------- thermal */
+
+/* thermalUpload (aeros/src/thermal_lbm.h:67): populations to g_eq(T, 0). */
+int aeros_lbm_thermal_upload(aeros_lbm* h, const float* T);
+/* thermalResetField (aeros/src/thermal_lbm.h:64). */
+int aeros_lbm_thermal_reset(aeros_lbm* h);
+
+/* ---------------------------------------------------------------- hooks */
+
+/* lbmEquilibriumMoments (aeros/src/lbm.h:106-107, aeros/src/lbm.cpp:249-266),
+ * evaluated by the device's equilibrium code.  Returns 1 on success, 0 on a
+ * non-finite input or null output pointer (the reference's bool). */
+int aeros_lbm_equilibrium_moments(float rho, float ux, float uy, float uz,
+                                  float* sum_feq, float* mom_x, float* mom_y, float* mom_z);
+
+/* ---------------------------------------------------------------- voxels */
+
+/* Solid mask by the 3-ray majority vote at cell centres
+ * (meshbvh::BVH::inside, aeros/src/mesh_bvh.h:168-185; loop of
+ * aeros/src/voxel_grid.cpp:337-351) on an arbitrary grid: centre of voxel
+ * (i,j,k) = box_min + (idx + 0.5) * cell.  tris holds 9 floats per triangle;
+ * non-finite and degenerate triangles are dropped as the reference does.
+ * out receives nx*ny*nz int32 (1 = solid). */
+int aeros_voxelize(const float* tris, int64_t ntris, int32_t nx, int32_t ny, int32_t nz,
+                   const float box_min[3], const float cell[3], int32_t* out);
+
+/* buildVoxelGrid's grid plan (aeros/src/voxel_grid.cpp:225-300): bbox plus
+ * 0.14*maxDim margin, nx = res, ny/nz proportional.  Fills dims and box. */
+int aeros_voxel_plan(const float bb_min[3], const float bb_max[3], float max_dim, int32_t res,
+                     int32_t dims[3], float box_min[3], float box_max[3]);
+
+/* ---------------------------------------------------------------- diagnostics */
+
+/* Kernel timing of the step kernel (no counterpart in the reference, whose
+ * only timer is the wall time of a stepLBMCPU call, aeros/src/lbm.cpp:916,1707):
+ * enable = 1 brackets every step-kernel launch with HIP events on the
+ * handle's stream and accumulates their elapsed time; enable = 0 stops and
+ * leaves the totals readable; enable = -1 resets them.  total_ms and
+ * launches may be NULL. */
+int aeros_lbm_kernel_timing(aeros_lbm* h, int32_t enable, double* total_ms, int64_t* launches);
+
+/* cudaConsumeError analogue (aeros/src/cuda_api.h:53-56): copies the last
+ * error message of this thread, clears it and returns its code (0 = none). */
+int aeros_lbm_last_error(char* buf, int32_t size);
+/* cudaProbeBackend analogue (aeros/src/cuda_api.h:29-46): enumerates devices
+ * and runs a test kernel. */
+int aeros_probe(aeros_backend_info* out);
+/* cudaReleaseAll analogue (aeros/src/cuda_api.h:51): frees library-wide
+ * device buffers (handles stay owned by their callers). */
+void aeros_release_all(void);
+
+/* The existing C-ABI hook the voxelizer calls (aeros/src/cuda_api.h:15-18,
+ * aeros/src/voxel_grid.cpp:552).  Kept with its name and signature; the
+ * signed distance field it carries is not used by the LBM and is ignored. */
+void setVoxelData(const int* voxel, const float* dist, int nx, int ny, int nz,
+                  float mnX, float mnY, float mnZ, float csX, float csY, float csZ);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AEROS_LBM_H */

@@ -1,0 +1,115 @@
+"""Measurements of the reference's headless suites, restated on field arrays.
+
+Each function takes a lattice object (the HIP ``aeros_lbm.Lattice`` or the
+CPU ``oracle.OracleLattice``; both expose step/field/scalars/cv_force) and
+returns the number the reference's test prints, so the same measurement can
+be compared against the golden vectors for either implementation.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+
+# ---------------------------------------------------------------- Poiseuille
+def channel_profile(L, force, tau, ny):
+    """runChannel's measurement (aeros/tests/physics_ref_headless.cpp:120-171)."""
+    nx = 8
+    nu = (tau - 0.5) / 3.0
+    H = float(ny - 2)
+    yc = 0.5 * (ny - 1)
+    u_max_ref = force * H * H / (8.0 * nu)
+    ux = L.field("ux").astype(np.float64)
+    zc = L.dims[2] // 2
+    err2 = ref2 = maxabs = flux = flux_ref = best = 0.0
+    for y in range(1, ny - 1):
+        u = float(np.sum(ux[zc, y, :], dtype=np.float64)) / nx
+        s = y - yc
+        u_ref = force / (2.0 * nu) * (0.25 * H * H - s * s)
+        err2 += (u - u_ref) ** 2
+        ref2 += u_ref ** 2
+        maxabs = max(maxabs, abs(u - u_ref))
+        flux += u
+        flux_ref += u_ref
+        if abs(s) <= 0.75:
+            best = max(best, u)
+    rho = L.field("rho").astype(np.float64)
+    solid = L.field("solid") > 0.5
+    ground = L.field("ground") > 0.5
+    fl = ~(solid | ground)
+    drift = abs(float(np.sum(rho[fl], dtype=np.float64)) / max(1, int(fl.sum())) - 1.0)
+    return dict(l2=math.sqrt(err2 / max(1e-30, ref2)), max_pct=100.0 * maxabs / max(1e-30, u_max_ref),
+                u_center=best, u_center_ref=u_max_ref, flux=flux, flux_ref=flux_ref, mass_drift=drift)
+
+
+# ---------------------------------------------------------------- thermal
+def harmonic_x(T, y, z):
+    """First x harmonic amplitude and phase (aeros/tests/thermal_headless.cpp:101-128)."""
+    nx = T.shape[2]
+    k = 2.0 * math.pi / nx
+    row = T[z, y, :].astype(np.float64)
+    re = im = 0.0
+    for x in range(nx):
+        a = k * x
+        re += row[x] * math.cos(a)
+        im -= row[x] * math.sin(a)
+    return 2.0 * math.sqrt(re * re + im * im) / nx, math.atan2(im, re)
+
+
+def perturbation_rms(T, solid, hot=1.0, cold=0.0):
+    """RMS deviation from the conduction profile (aeros/tests/thermal_headless.cpp:130-147)."""
+    nz, ny, nx = T.shape
+    span = float(ny - 2)
+    acc = 0.0
+    n = 0
+    for y in range(1, ny - 1):
+        s = (y - 0.5) / max(1.0, span)
+        lin = hot + (cold - hot) * s
+        m = ~solid[:, y, :]
+        d = T[:, y, :].astype(np.float64)[m] - lin
+        acc += float(np.sum(d * d))
+        n += int(m.sum())
+    return math.sqrt(acc / n) if n else 0.0
+
+
+def cosine_T(shape):
+    nz, ny, nx = shape
+    k = 2.0 * math.pi / nx
+    row = np.array([np.float32(0.5 + 0.5 * math.cos(k * x)) for x in range(nx)], dtype=np.float32)
+    return np.broadcast_to(row[None, None, :], shape).copy()
+
+
+# ---------------------------------------------------------------- drag
+def cd_series(L, u0, warm, meas, box=0.30):
+    """runCase's Cd / Cl over the measured steps (aeros/tests/validation_headless.cpp:137-160)."""
+    L.step(warm)
+    area = L.projected_area(0)
+    q = np.float32(0.5) * np.float32(u0) * np.float32(u0) * np.float32(area)
+    cds, cls = [], []
+    for _ in range(meas):
+        L.step(1)
+        fx, fy, fz = L.cv_force(box, box, box)
+        cds.append(float(np.float32(fx) / q))
+        cls.append(float(np.float32(fy) / q))
+    cd = np.array(cds, dtype=np.float32)
+    cl = np.array(cls, dtype=np.float32)
+    cd_mean = 0.0
+    for v in cds:
+        cd_mean += v
+    return dict(area=area, cd=float(np.float32(cd_mean / len(cds))), cd_amp=float(cd.max() - cd.min()),
+                cl_amp=float(cl.max() - cl.min()))
+
+
+def host_checksums(L):
+    """The bench's sequential double sums (aeros/tests/bench_lbm.cpp:103-107)."""
+    rho = L.field("rho").ravel()
+    ux, uy, uz = (L.field(k).ravel() for k in ("ux", "uy", "uz"))
+    su3 = (ux + uy + uz).astype(np.float32)   # float sum per cell, as the reference does
+    s_rho = 0.0
+    s_u = 0.0
+    # sequential double accumulation (math.fsum would not be the same operation)
+    for a, b in zip(rho.tolist(), su3.tolist()):
+        s_rho += a
+        s_u += b
+    return s_rho, s_u

@@ -1,0 +1,141 @@
+"""GPU parity: the HIP solver against the CPU oracle on the same inputs.
+
+Float paths must agree bit for bit (fields and post-stream populations);
+the FP64 path to 1e-12 relative.  All comparisons go through the C ABI.
+"""
+import copy
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import aeros_lbm as A
+import workloads as W
+from measures import host_checksums
+
+pytestmark = pytest.mark.gpu
+
+GOLD = Path(__file__).resolve().parent / "golden"
+FIELDS = ("rho", "ux", "uy", "uz", "tke", "strain", "ux_world", "uy_world", "uz_world", "pressure")
+
+
+def make_pair(oracle, **over):
+    cfg_g = A.default_config(**over)
+    cfg_o = A.default_config(**over)
+    print(f"case {sorted(k for k in over if k != 'voxels')}", flush=True)
+    G = A.Lattice(cfg_g)
+    O = oracle.OracleLattice(cfg_o)
+    assert G.dims == O.dims
+    return G, O
+
+
+def assert_same(G, O, names=FIELDS, rtol=0.0):
+    for name in names:
+        g = G.field(name, dtype=np.float64)
+        o = O.field(name, dtype=np.float64)
+        if rtol == 0.0:
+            bad = np.argwhere(g != o)
+            assert bad.size == 0, f"{name}: {len(bad)} cells differ, first {bad[:3].tolist()} gpu {g[tuple(bad[0])]!r} oracle {o[tuple(bad[0])]!r}"
+        else:
+            np.testing.assert_allclose(g, o, rtol=rtol, atol=rtol * max(1.0, float(np.abs(o).max())), err_msg=name)
+
+
+def test_probe():
+    info = A.probe()
+    assert info["usable"] == 1, info
+    assert info["arch_name"].startswith("gfx950"), info
+
+
+@pytest.mark.parametrize("les", [1, 0])
+def test_bench_sphere_bitwise(oracle, les):
+    G, O = make_pair(oracle, **W.bench_sphere(12, les=bool(les)))
+    for n in (1, 29, 20):
+        G.step(n)
+        O.step(n)
+        assert_same(G, O)
+    np.testing.assert_array_equal(G.populations(), O.populations())
+    sg, so = G.scalars(), O.scalars()
+    for k in ("step", "avg_rho", "avg_ke", "max_u_lb", "max_u_world", "tke", "convergence", "neg_pops",
+              "bad_pops", "u_limited", "pressure_min", "pressure_max", "residual_rho"):
+        assert sg[k] == so[k], (k, sg[k], so[k])
+
+
+def test_bench_published_checksum(oracle):
+    """80x72x72 bench, LES, 30+120 steps: the reference's uncontracted checksum."""
+    ref = json.loads((GOLD / "ref_bench.json").read_text())
+    case = [c for c in ref["cases"] if c["N"] == 32 and c["les"] == 1 and c["thermal"] == 0][0]
+    G = A.Lattice(A.default_config(**W.bench_sphere(32)))
+    assert list(G.dims) == case["grid"]
+    G.step(30)
+    G.step(case["steps"])
+    r, u = G.checksums()
+    hr, hu = host_checksums(G)
+    assert r == hr                                    # exact fixed-point sum == sequential double sum
+    assert "%.12g" % r == case["rho_sum"]
+    assert "%.12g" % hu == case["u_sum"]
+    assert abs(u - hu) <= 1e-12 * abs(hu)
+
+
+def test_fp64_against_fp64_oracle(oracle):
+    G, O = make_pair(oracle, **W.bench_sphere(12), precision="fp64")
+    G.step(120)
+    O.step(120)
+    assert_same(G, O, names=("rho", "ux", "uy", "uz", "tke"), rtol=1e-12)
+    rg, _ = G.checksums()
+    ro, _ = O.checksums()
+    assert abs(rg - ro) <= 1e-12 * abs(ro)
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+def test_16bit_storage_matches_quantised_oracle(oracle, prec):
+    G, O = make_pair(oracle, **W.bench_sphere(12), precision=prec)
+    G.step(40)
+    O.step(40)
+    assert_same(G, O, names=("rho", "ux", "uy", "uz", "tke", "strain"))
+
+
+def test_thermal_coupled_bitwise(oracle):
+    G, O = make_pair(oracle, **W.bench_sphere(12, les=True, thermal=True))
+    G.step(25)
+    O.step(25)
+    assert_same(G, O, names=("rho", "ux", "uy", "uz", "tke", "strain", "T", "buoyancy"))
+    sg, so = G.scalars(), O.scalars()
+    assert sg["nusselt"] == so["nusselt"] and sg["thermal_mean_t"] == so["thermal_mean_t"]
+
+
+def test_channel_trt_guo_periodic_bitwise(oracle):
+    G, O = make_pair(oracle, **W.channel(34, 0.65, 7.8e-6, trt=True))
+    G.step(300)
+    O.step(300)
+    assert_same(G, O, names=("rho", "ux", "uy", "uz"))
+    np.testing.assert_array_equal(G.populations(), O.populations())
+
+
+def test_thermal_walls_dirichlet_bitwise(oracle):
+    cfg = W.thermal_box(32, 18, 8, walls_y=True, rayleigh=2500.0, fixed_wall_temps=True, perturbation=0.02)
+    G, O = make_pair(oracle, **cfg)
+    G.step(200)
+    O.step(200)
+    assert_same(G, O, names=("rho", "ux", "uy", "uz", "T", "buoyancy"))
+    assert G.scalars()["nusselt"] == O.scalars()["nusselt"]
+
+
+def test_cv_force_and_area(oracle):
+    G, O = make_pair(oracle, **W.validate_sphere(24, 10))
+    G.step(60)
+    O.step(60)
+    fg, fo = np.array(G.cv_force()), np.array(O.cv_force())
+    np.testing.assert_allclose(fg, fo, rtol=1e-12, atol=1e-15)
+    assert G.projected_area(0) == O.projected_area(0)
+
+
+def test_rerun_is_deterministic():
+    cfg = W.bench_sphere(16)
+    a = A.Lattice(A.default_config(**cfg))
+    b = A.Lattice(A.default_config(**cfg))
+    a.step(50)
+    b.step(17)
+    b.step(33)
+    for k in ("rho", "ux", "uy", "uz"):
+        np.testing.assert_array_equal(a.field(k), b.field(k))
