@@ -181,8 +181,12 @@ def c2_config(voxelize_fn, nx=256, ny=128, nz=104, scale=1.0, seed=12345, re=Non
     """256x128x104, TRT (magic 3/16) + Smagorinsky Cs 0.1, U0 0.08, no body force, an ellipsoid
     STL (semi-axes 30/12/12 cells, ~200k triangles, vertex jitter 0.05 cells, seeded) voxelized
     by the BVH three-ray vote at cell centres.  tau is stated: 0.5 + 3 U0 D / Re with D = 24 (the
-    minor-axis diameter) and Re = 300 -> 0.5192; scale=2 gives the 512x256x256 case with Re 1000."""
-    nx, ny, nz = int(nx * scale), int(ny * scale), int(nz * scale)
+    minor-axis diameter) and Re = 300 -> 0.5192.  scale=2 scales the body and the x/y extents and
+    gives the 512x256x256 (33.5M-cell) case with Re 1000 -> tau 0.51152 (nz is 256 there)."""
+    if scale != 1.0 and nz == 104:
+        nx, ny, nz = int(nx * scale), int(ny * scale), 256
+    else:
+        nx, ny, nz = int(nx * scale), int(ny * scale), int(nz * scale)
     semi = (30.0 * scale, 12.0 * scale, 12.0 * scale)
     centre = (nx * 0.3, ny * 0.5, nz * 0.5)
     cfg = dict(**lattice_box(nx, ny, nz), voxel_resolution=nx, max_dim=float(2 * semi[0]),

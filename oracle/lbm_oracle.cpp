@@ -322,6 +322,7 @@ public:
     float reLB = 0, rePhys = 0, blockage = 0;
     // scalars
     int curStep = 0; bool converged = false; int resets = 0;
+    float lastDivergedRho = 0.0f; int lastDivergedStep = -1;
     float avgRho = 1, avgKE = 0, maxVelLB = 0, maxVelW = 0, tkeMean = 0, conv = 0;
     long long negPops = 0, badPops = 0, rhoOut = 0, rhoNonFin = 0, uLim = 0, pBad = 0;
     float prevKinetic = 0;
@@ -988,7 +989,7 @@ public:
             conv = std::fabs(avgKE - prevKinetic);
             prevKinetic = avgKE;
             converged = conv < cfg.convergence_threshold;
-            if (!std::isfinite(avgRho) || avgRho > 1.2f || avgRho < 0.8f) { diverged = true; break; }
+            if (!std::isfinite(avgRho) || avgRho > 1.2f || avgRho < 0.8f) { diverged = true; lastDivergedRho = avgRho; lastDivergedStep = curStep; break; }
         }
         if (diverged) reset_flow(inUx, inUy, inUz);
         post_pass();
@@ -1314,6 +1315,12 @@ int ora_equilibrium_moments(float rho, float ux, float uy, float uz, float* s, f
     }
     *s = m; *mx = px; *my = py; *mz = pz;
     return 1;
+}
+
+// avgRho and step of the last divergence reset (diagnostic of the reset path)
+float ora_last_divergence(void* p, int* step) {
+    Handle* h = (Handle*)p;
+    return ORA_DISPATCH(h, (*step = S.lastDivergedStep, S.lastDivergedRho));
 }
 
 float ora_q_half(float x) { return q_half(x); }

@@ -59,6 +59,7 @@ def load() -> C.CDLL:
         "ora_voxelize": (C.c_int, [P(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                    P(C.c_float), P(C.c_float), P(C.c_int32)]),
         "ora_threads": (C.c_int, [C.c_int]),
+        "ora_last_divergence": (C.c_float, [vp, P(C.c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -171,6 +172,12 @@ class OracleLattice:
 
     def thermal_reset(self):
         self.lib.ora_thermal_reset(self.h)
+
+    def last_divergence(self):
+        """(step, avg rho) of the last divergence reset, step -1 if none."""
+        st = C.c_int(-1)
+        r = self.lib.ora_last_divergence(self.h, C.byref(st))
+        return st.value, r
 
 
 def equilibrium_moments(rho, ux, uy, uz):

@@ -130,6 +130,10 @@ def lbm_suite() -> dict:
      d["les_convergence"]) = grab(r"\[турбулентность\] rho=" + NUM + r"\s+TKE=" + NUM + r" м2/с2\s+vmax=" + NUM +
                                   r" м/с\s+p=\[" + NUM + r", " + NUM + r"\] Па\s+отриц\.популяций=" + NUM +
                                   r"\s+замен=" + NUM + r"\s+сходимость=" + NUM, t, 8)
+    d["lam_avg_rho"], d["lam_vmax"] = grab(r"\[без турбулент\.\] rho=" + NUM + r"\s+vmax=" + NUM, t, 2)
+    d["outlet_off_avg_rho"], d["outlet_on_avg_rho"], d["outlet_off_vmax"], d["outlet_on_vmax"] = grab(
+        r"конвективный выход меняет поле: rho " + NUM + r" против " + NUM + r", vmax " + NUM + r" против " + NUM, t, 4)
+    d["ground_off_vmax"], d["ground_on_vmax"] = grab(r"земля меняет решение: vmax " + NUM + r" против " + NUM, t, 2)
     d["thread_rho_sum"], d["thread_u_sum"] = grab(r"не зависит от числа потоков: rho " + NUM + r" против [\d.]+ \(отклонение [\w.+-]+\), u " + NUM, t, 2)
     d["padded_grid"] = grab(r"домен: размер (\d+)x(\d+)x(\d+)", t, 3, int)
     d["padded_solid_cells"] = grab(r"маска тела сохранена \((\d+) твёрдых", t, 1, int)

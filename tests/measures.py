@@ -82,8 +82,12 @@ def cosine_T(shape):
 
 # ---------------------------------------------------------------- drag
 def cd_series(L, u0, warm, meas, box=0.30):
-    """runCase's Cd / Cl over the measured steps (aeros/tests/validation_headless.cpp:137-160)."""
-    L.step(warm)
+    """runCase's Cd / Cl over the measured steps (aeros/tests/validation_headless.cpp:137-160).
+
+    One step per call, as runCase calls stepLBMCPU(1): a divergence reset ends
+    the call it happens in, so the call pattern is part of the semantics."""
+    for _ in range(warm):
+        L.step(1)
     area = L.projected_area(0)
     q = np.float32(0.5) * np.float32(u0) * np.float32(u0) * np.float32(area)
     cds, cls = [], []

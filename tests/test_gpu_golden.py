@@ -1,0 +1,142 @@
+"""GPU: the HIP solver against the reference's own published and measured numbers.
+
+Same fixtures as tests/test_oracle_golden.py, but the solver under test is
+the HIP library behind the C ABI: bench checksums (all cases), Poiseuille,
+Rayleigh-Benard Nu 1.457, sphere Cd 1.128, the divergence-reset cylinder,
+the curl-noise inflow and the suite's boundary switches.
+"""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import aeros_lbm as A
+import workloads as W
+from measures import cd_series, channel_profile, perturbation_rms, host_checksums
+
+pytestmark = pytest.mark.gpu
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def gold(name):
+    return json.loads((GOLD / f"ref_{name}.json").read_text())
+
+
+def same(fmt, value, ref):
+    assert fmt % value == fmt % ref, f"{fmt % value} != {fmt % ref}"
+
+
+def lattice(**over):
+    print(f"case {sorted(k for k in over if k != 'voxels')}", flush=True)
+    return A.Lattice(A.default_config(**over))
+
+
+@pytest.mark.parametrize("case", gold("bench")["cases"], ids=lambda c: f"N{c['N']}-s{c['steps']}-th{c['thermal']}-les{c['les']}")
+def test_bench_checksums_gpu(case):
+    L = lattice(**W.bench_sphere(case["N"], les=bool(case["les"]), thermal=bool(case["thermal"])))
+    assert list(L.dims) == case["grid"]
+    L.step(case["warmup"] + case["steps"])
+    r, u = L.checksums()
+    assert "%.12g" % r == case["rho_sum"]
+    assert "%.12g" % u == case["u_sum"]
+
+
+def test_poiseuille_gpu():
+    phys = gold("physref")
+    for trt in (False, True):
+        L = lattice(**W.channel(34, 0.65, 7.8e-6, trt=trt))
+        L.step(40000)
+        r = channel_profile(L, 7.8e-6, 0.65, 34)
+        if trt:
+            same("%.3f", r["l2"] * 100, phys["trt_l2_pct"])
+            same("%.6f", r["u_center"], phys["trt_u_center"])
+        else:
+            same("%.6f", r["u_center"], phys["poiseuille_u_center"])
+            same("%.3f", r["l2"] * 100, phys["poiseuille_l2_pct"])
+            same("%.3f", r["max_pct"], phys["poiseuille_max_pct"])
+            same("%.6f", r["flux"], phys["poiseuille_flux"])
+            same("%.2e", r["mass_drift"], phys["poiseuille_mass_drift"])
+
+
+def test_rayleigh_benard_nu_gpu():
+    th = gold("thermal")
+    L = lattice(**W.thermal_box(32, 18, 8, walls_y=True, rayleigh=2500.0, fixed_wall_temps=True, perturbation=0.02))
+    solid = L.field("solid") > 0.5
+    L.step(6000)
+    a = perturbation_rms(L.field("T"), solid)
+    L.step(18000)
+    b = perturbation_rms(L.field("T"), solid)
+    same("%.5f", a, th["rb2500_amp_6000"])
+    same("%.5f", b, th["rb2500_amp_24000"])
+    same("%.3f", L.scalars()["nusselt"], th["rb2500_nu"])
+
+
+def test_sphere_cd_gpu():
+    val = gold("validate")
+    L = lattice(**W.validate_sphere(48, 20))
+    assert list(L.dims) == val["sphere_grid"]
+    r = cd_series(L, 0.08, 500, 50)
+    same("%.0f", r["area"], val["sphere_area"])
+    same("%.3f", r["cd"], val["sphere_cd"])
+    same("%.4f", r["cd_amp"], val["sphere_cd_amp"])
+
+
+def test_cylinder_divergence_reset_gpu():
+    val = gold("validate")
+    over = W.validate_sphere(36, 16, pad=0.15)
+    over["voxels"] = W.cylinder_voxels(36, -3.0, 3.0, float(np.float32(3.0) * np.float32(16) / np.float32(36)))
+    L = lattice(**over)
+    r = cd_series(L, 0.08, 900, 400)
+    assert L.scalars()["resets"] == 1
+    same("%.3f", r["cd"], val["cylinder_cd"])
+    same("%.3f", r["cl_amp"], val["cylinder_cl_amp"])
+
+
+def test_curl_noise_suite_gpu():
+    lbm = gold("lbm")
+    L = lattice(**W.lbm_suite_sphere(les=True))
+    for _ in range(30):
+        L.step(1)
+    r, u = L.checksums()
+    same("%.10g", r, lbm["thread_rho_sum"])
+    same("%.10g", u, lbm["thread_u_sum"])
+    for _ in range(90):
+        L.step(1)
+    s = L.scalars()
+    same("%.6f", s["avg_rho"], lbm["les_avg_rho"])
+    same("%.4e", s["tke"], lbm["les_tke"])
+    same("%.3f", s["max_u_world"], lbm["les_vmax"])
+    same("%.2f", s["pressure_min"], lbm["les_pmin"])
+    same("%.2f", s["pressure_max"], lbm["les_pmax"])
+    same("%.3e", s["convergence"], lbm["les_convergence"])
+
+
+@pytest.mark.parametrize("variant", ["outlet_off", "ground_on"])
+def test_boundary_switches_gpu(variant):
+    lbm = gold("lbm")
+    over = dict(W.lbm_suite_sphere(les=True))
+    if variant == "outlet_off":
+        over["convective_outlet"] = 0
+    else:
+        over.update(use_ground=1, ground_height=0.0)
+    L = lattice(**over)
+    for _ in range(60):
+        L.step(1)
+    s = L.scalars()
+    if variant == "ground_on":
+        same("%.4f", s["max_u_world"], lbm["ground_on_vmax"])
+    else:
+        same("%.6f", s["avg_rho"], lbm["outlet_off_avg_rho"])
+        same("%.4f", s["max_u_world"], lbm["outlet_off_vmax"])
+
+
+def test_equilibrium_moments_gpu():
+    for rho, ux, uy, uz in [(1.0, 0.0, 0.0, 0.0), (0.7, 0.04, -0.03, 0.02), (1.35, -0.02, 0.05, 0.01),
+                            (1.1, 0.28, 0.0, 0.0), (1.6, 0.0, 0.0, -0.08)]:
+        ok, (m, px, py, pz) = A.equilibrium_moments(rho, ux, uy, uz)
+        assert ok
+        assert abs(m - rho) <= 3e-5 * max(1.0, rho)
+        assert abs(px - rho * ux) <= 3e-5 and abs(py - rho * uy) <= 3e-5 and abs(pz - rho * uz) <= 3e-5
+    assert not A.equilibrium_moments(1.0, float("nan"), 0.0, 0.0)[0]

@@ -1,0 +1,301 @@
+"""CPU: the oracle against the reference's own golden vectors (tests/golden/ref_*.json).
+
+The fixtures were produced by tests/golden/make_golden.py from the
+reference's bench and headless suites built from its sources.  Printed
+numbers are compared at the precision the reference prints them, formatted
+the same way, so the oracle must reproduce them digit for digit.
+"""
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import workloads as W
+from measures import channel_profile, cosine_T, harmonic_x, perturbation_rms, cd_series
+
+GOLD = Path(__file__).resolve().parent / "golden"
+
+
+def gold(name):
+    return json.loads((GOLD / f"ref_{name}.json").read_text())
+
+
+def same(fmt, value, ref):
+    assert fmt % value == fmt % ref, f"{fmt % value} != {fmt % ref}"
+
+
+# ---------------------------------------------------------------- bench checksums (G1, G3)
+BENCH = gold("bench")["cases"]
+
+
+@pytest.mark.parametrize("case", BENCH, ids=lambda c: f"N{c['N']}-s{c['steps']}-th{c['thermal']}-les{c['les']}")
+def test_bench_checksums(oracle, case):
+    if case["N"] >= 64:
+        pytest.importorskip("numpy")
+    L = oracle.OracleLattice(oracle.default_config(**W.bench_sphere(case["N"], les=bool(case["les"]),
+                                                                   thermal=bool(case["thermal"]))))
+    assert list(L.dims) == case["grid"]
+    for _ in range(case["warmup"] + case["steps"]):
+        L.step(1)                        # the bench calls stepLBMCPU(1) per step
+    r, u = L.checksums()
+    assert "%.12g" % r == case["rho_sum"]
+    assert "%.12g" % u == case["u_sum"]
+
+
+def test_published_checksum_is_the_fma_build():
+    """The published 414728.371435 comes from an FMA-contracted build; the
+    uncontracted reference (and the oracle) differ from it by ~1.5e-8 relative."""
+    g = gold("bench")
+    pub = float(g["published_fma_build"]["rho_sum"])
+    plain = float([c for c in g["cases"] if c["N"] == 32 and c["les"] == 1 and c["thermal"] == 0][0]["rho_sum"])
+    assert "%.12g" % pub == "414728.371435"
+    assert 0 < abs(pub - plain) / pub < 1e-7
+
+
+# ---------------------------------------------------------------- solver suite (G2, G7)
+LBMG = gold("lbm")
+
+
+def test_suite_sphere_les_and_laminar(oracle):
+    """aeros/tests/lbm_physics_headless.cpp:71-112, 174-184: curl-noise inflow (inletTurbulence 0.02)."""
+    L = oracle.OracleLattice(oracle.default_config(**W.lbm_suite_sphere(les=True)))
+    for _ in range(120):
+        L.step(1)
+    s = L.scalars()
+    same("%.6f", s["avg_rho"], LBMG["les_avg_rho"])
+    same("%.4e", s["tke"], LBMG["les_tke"])
+    same("%.3f", s["max_u_world"], LBMG["les_vmax"])
+    same("%.2f", s["pressure_min"], LBMG["les_pmin"])
+    same("%.2f", s["pressure_max"], LBMG["les_pmax"])
+    same("%.3e", s["convergence"], LBMG["les_convergence"])
+    assert s["neg_pops"] == 0 and s["bad_pops"] == 0
+    L2 = oracle.OracleLattice(oracle.default_config(**W.lbm_suite_sphere(les=False)))
+    for _ in range(120):
+        L2.step(1)
+    s2 = L2.scalars()
+    same("%.6f", s2["avg_rho"], LBMG["lam_avg_rho"])
+    same("%.3f", s2["max_u_world"], LBMG["lam_vmax"])
+
+
+def test_suite_thread_checksum(oracle):
+    """G2: 30 LES steps of the suite sphere; rho/u sums printed at 10 digits."""
+    L = oracle.OracleLattice(oracle.default_config(**W.lbm_suite_sphere(les=True)))
+    for _ in range(30):
+        L.step(1)
+    r, u = L.checksums()
+    same("%.10g", r, LBMG["thread_rho_sum"])
+    same("%.10g", u, LBMG["thread_u_sum"])
+
+
+@pytest.mark.parametrize("variant", ["outlet_off", "outlet_on", "ground_on"])
+def test_suite_boundary_switches(oracle, variant):
+    """run60 of aeros/tests/lbm_physics_headless.cpp:424-455."""
+    over = dict(W.lbm_suite_sphere(les=True))
+    if variant == "outlet_off":
+        over["convective_outlet"] = 0
+    if variant == "ground_on":
+        over.update(use_ground=1, ground_height=0.0)
+    L = oracle.OracleLattice(oracle.default_config(**over))
+    for _ in range(60):
+        L.step(1)
+    s = L.scalars()
+    if variant == "ground_on":
+        same("%.4f", s["max_u_world"], LBMG["ground_on_vmax"])
+    else:
+        same("%.6f", s["avg_rho"], LBMG[f"{variant}_avg_rho"])
+        same("%.4f", s["max_u_world"], LBMG[f"{variant}_vmax"])
+
+
+def test_padded_domain_geometry(oracle):
+    """domainPadding 1.0: 2L upstream, 4L downstream, 2.5L lateral; mask preserved."""
+    over = dict(W.lbm_suite_sphere())
+    over["domain_padding"] = 1.0
+    L = oracle.OracleLattice(oracle.default_config(**over))
+    assert list(L.dims) == LBMG["padded_grid"]
+    assert int(L.field("solid").sum()) == LBMG["padded_solid_cells"]
+
+
+EQ_CASES = [(1.00, 0.00, 0.00, 0.00), (1.00, 0.05, 0.00, 0.00), (0.70, 0.04, -0.03, 0.02),
+            (1.35, -0.02, 0.05, 0.01), (0.55, 0.10, 0.10, 0.00), (1.60, 0.00, 0.00, -0.08),
+            (1.10, 0.28, 0.00, 0.00)]
+
+
+@pytest.mark.parametrize("rho,ux,uy,uz", EQ_CASES)
+def test_equilibrium_moments_closed_form(oracle, rho, ux, uy, uz):
+    """G7: sum feq = rho, sum feq c = rho u within 3e-5 (aeros/tests/lbm_physics_headless.cpp:293-319)."""
+    ok, (m, px, py, pz) = oracle.equilibrium_moments(rho, ux, uy, uz)
+    assert ok
+    assert abs(m - rho) <= 3e-5 * max(1.0, rho)
+    for p, u in ((px, ux), (py, uy), (pz, uz)):
+        assert abs(p - rho * u) <= 3e-5
+
+
+def test_equilibrium_moments_rejects_nonfinite(oracle):
+    assert not oracle.equilibrium_moments(1.0, float("nan"), 0.0, 0.0)[0]
+    assert not oracle.equilibrium_moments(float("nan"), 0.0, 0.0, 0.0)[0]
+
+
+# ---------------------------------------------------------------- Poiseuille (G4)
+PHYS = gold("physref")
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("trt", [False, True])
+def test_poiseuille(oracle, trt):
+    L = oracle.OracleLattice(oracle.default_config(**W.channel(34, 0.65, 7.8e-6, trt=trt)))
+    L.step(40000)
+    r = channel_profile(L, 7.8e-6, 0.65, 34)
+    if trt:
+        same("%.3f", r["l2"] * 100, PHYS["trt_l2_pct"])
+        same("%.6f", r["u_center"], PHYS["trt_u_center"])
+        same("%.2e", r["mass_drift"], PHYS["trt_mass_drift"])
+    else:
+        same("%.6f", r["u_center"], PHYS["poiseuille_u_center"])
+        same("%.6f", r["u_center_ref"], PHYS["poiseuille_u_exact"])
+        same("%.3f", r["l2"] * 100, PHYS["poiseuille_l2_pct"])
+        same("%.3f", r["max_pct"], PHYS["poiseuille_max_pct"])
+        same("%.6f", r["flux"], PHYS["poiseuille_flux"])
+        same("%.2e", r["mass_drift"], PHYS["poiseuille_mass_drift"])
+
+
+@pytest.mark.slow
+def test_grid_convergence_and_viscosity(oracle):
+    ls = []
+    for ny, force, steps in ((10, 7.8e-6 * 16, 4000), (18, 7.8e-6 * 4, 15000)):
+        L = oracle.OracleLattice(oracle.default_config(**W.channel(ny, 0.65, force)))
+        L.step(steps)
+        ls.append(channel_profile(L, force, 0.65, ny)["l2"])
+    same("%.3f", ls[0] * 100, PHYS["l2_h8_pct"])
+    same("%.3f", ls[1] * 100, PHYS["l2_h16_pct"])
+    uc = []
+    for steps in (5000, 6000):
+        L = oracle.OracleLattice(oracle.default_config(**W.channel(34, 0.65, 0.0)))
+        L.step(steps)
+        uc.append(channel_profile(L, 0.0, 0.65, 34)["u_center"])
+    lam = math.log(uc[0] / uc[1]) / 1000.0
+    same("%.3e", lam, PHYS["decay_lambda"])
+    nu = (0.65 - 0.5) / 3.0
+    assert abs(lam - nu * math.pi ** 2 / 32 ** 2) / (nu * math.pi ** 2 / 32 ** 2) < 0.03
+
+
+# ---------------------------------------------------------------- thermal (G6)
+TH = gold("thermal")
+
+
+def test_thermal_diffusion_and_advection(oracle):
+    k = 2.0 * math.pi / 32
+    for u0, steps, key in ((0.0, 1500, "diffusion"), (0.05, 1200, "advection")):
+        L = oracle.OracleLattice(oracle.default_config(**W.thermal_box(32, 16, 8, u0=u0)))
+        L.thermal_upload(cosine_T(L.shape))
+        T0 = L.field("T")
+        A0, ph0 = harmonic_x(T0, 8, 4)
+        L.step(steps)
+        T1 = L.field("T")
+        A1, ph1 = harmonic_x(T1, 8, 4)
+        if key == "diffusion":
+            alpha = -math.log(max(1e-12, A1 / A0)) / (k * k * steps)
+            same("%.6f", A1, TH["diffusion_A1"])
+            same("%.6f", alpha, TH["diffusion_alpha"])
+            same("%.6f", L.scalars()["thermal_alpha"], TH["diffusion_alpha_set"])
+        else:
+            ux = L.field("ux").astype(np.float64)
+            u_act = float(np.sum(ux)) / ux.size
+            dphi = ph1 - (ph0 - k * u_act * steps)
+            while dphi > math.pi:
+                dphi -= 2 * math.pi
+            while dphi < -math.pi:
+                dphi += 2 * math.pi
+            u_meas = u_act * (1.0 + dphi / (k * u_act * steps))
+            same("%.6f", u_meas, TH["advection_u_phase"])
+            same("%.5f", dphi, TH["advection_phase_residual"])
+            same("%.5f", A1, TH["advection_A1"])
+
+
+def test_thermal_conduction_nu_one(oracle):
+    L = oracle.OracleLattice(oracle.default_config(**W.thermal_box(24, 18, 8, walls_y=True, fixed_wall_temps=True)))
+    L.thermal_upload(np.full(L.shape, 0.5, dtype=np.float32))
+    L.step(6000)
+    T = L.field("T").astype(np.float64)
+    solid = L.field("solid") > 0.5
+    worst = 0.0
+    for z in range(8):
+        for y in range(1, 17):
+            if solid[z, y, 12]:
+                continue
+            lin = 1.0 - (y - 0.5) / 16.0
+            worst = max(worst, abs(T[z, y, 12] - lin))
+    same("%.5f", worst, TH["conduction_max_dev"])
+    same("%.4f", L.scalars()["nusselt"], TH["conduction_nu"])
+
+
+@pytest.mark.slow
+@pytest.mark.parametrize("ra", [1000, 1700, 2500])
+def test_rayleigh_benard(oracle, ra):
+    """G6: amplitudes at 6000 and 24000 steps and Nu (Nu 1.457 at Ra 2500)."""
+    L = oracle.OracleLattice(oracle.default_config(**W.thermal_box(
+        32, 18, 8, walls_y=True, rayleigh=float(ra), fixed_wall_temps=True, perturbation=0.02)))
+    solid = L.field("solid") > 0.5
+    L.step(6000)
+    a = perturbation_rms(L.field("T"), solid)
+    L.step(18000)
+    b = perturbation_rms(L.field("T"), solid)
+    same("%.5f", a, TH[f"rb{ra}_amp_6000"])
+    same("%.5f", b, TH[f"rb{ra}_amp_24000"])
+    same("%.3f", L.scalars()["nusselt"], TH[f"rb{ra}_nu"])
+
+
+def test_buoyancy_sign(oracle):
+    L = oracle.OracleLattice(oracle.default_config(**W.thermal_box(8, 8, 8, rayleigh=1.0)))
+    T = np.full(L.shape, 0.5, dtype=np.float32)
+    T[0, 5, 4] = 1.0
+    T[0, 2, 4] = 0.0
+    L.thermal_upload(T)
+    L.step(1)
+    B = L.field("buoyancy")
+    same("%+.3e", B[0, 5, 4], TH["buoyancy_hot"])
+    same("%+.3e", B[0, 2, 4], TH["buoyancy_cold"])
+
+
+# ---------------------------------------------------------------- sphere drag (G5) and reset path
+VAL = gold("validate")
+
+
+@pytest.mark.slow
+def test_sphere_cd(oracle):
+    L = oracle.OracleLattice(oracle.default_config(**W.validate_sphere(48, 20)))
+    assert list(L.dims) == VAL["sphere_grid"]
+    r = cd_series(L, 0.08, 500, 50)
+    same("%.0f", r["area"], VAL["sphere_area"])
+    same("%.3f", r["cd"], VAL["sphere_cd"])
+    same("%.4f", r["cd_amp"], VAL["sphere_cd_amp"])
+
+
+@pytest.mark.slow
+def test_cylinder_with_divergence_reset(oracle):
+    """The cylinder case diverges once and is reset (aeros/src/lbm.cpp:1574-1600)."""
+    vox = W.cylinder_voxels(36, -3.0, 3.0, float(np.float32(3.0) * np.float32(16) / np.float32(36)))
+    over = W.validate_sphere(36, 16, pad=0.15)
+    over["voxels"] = vox
+    L = oracle.OracleLattice(oracle.default_config(**over))
+    r = cd_series(L, 0.08, 900, 400)
+    assert L.scalars()["resets"] >= 1
+    same("%.3f", r["cd"], VAL["cylinder_cd"])
+    same("%.3f", r["cl_amp"], VAL["cylinder_cl_amp"])
+
+
+# ---------------------------------------------------------------- quantisers of the 16-bit modes
+def test_quantisers_round_to_nearest_even(oracle):
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.normal(0, 0.05, 2000), rng.normal(0, 1e-5, 500), [0.0, -0.0, 1e-8, 65504.0, 65520.0]])
+    for x in xs.astype(np.float32):
+        assert oracle.q_half(float(x)) == float(np.float32(np.float16(x))), x
+        # bfloat16: the nearer of the two bracketing bfloat16 values, ties to the even one
+        u = int(np.array([x], dtype=np.float32).view(np.uint32)[0])
+        lo_bits = u & 0xFFFF0000
+        cand = [lo_bits, lo_bits + 0x10000]
+        vals = [float(np.array([c], dtype=np.uint32).view(np.float32)[0]) for c in cand]
+        d = [abs(float(x) - v) for v in vals]
+        pick = vals[0] if d[0] < d[1] else (vals[1] if d[1] < d[0] else vals[(cand[0] >> 16) & 1])
+        assert oracle.q_bf16(float(x)) == pick, x
