@@ -115,7 +115,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
     hipStream_t st;
     int64_t n = 0;
     int grid = 1;
-    DevBuf f[2], mask, solid_u8, u[2][3], fld[6], world[4], g[2], T[2], buoy[2], parts, status, prev_rho, scratch;
+    DevBuf f[2], mask, solid_u8, u[2], fld[6], world[4], g[2], T[2], buoy[2], parts, status, prev_rho, scratch;
     bool pre = true;          // buffers hold pre-collision populations (after init / reset)
     int cur = 0;              // lbmCurrentStep (host mirror)
     int resets = 0;
@@ -156,6 +156,12 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         if (!std::isfinite(tau0) || tau0 < 0.51f) tau0 = 0.55f;
         if (tau0 > 1.5f) tau0 = 1.5f;
         P.tau0 = (R)tau0;
+        P.inv_tau0 = R(1) / P.tau0;
+        {
+            const R mm = P.inv_tau0 - R(0.5);
+            const R den = (std::fabs(mm) > R(1e-6)) ? (R(0.1875) / mm + R(0.5)) : R(1);
+            P.s_minus0 = R(1) / (den > R(0.05) ? den : R(0.05));
+        }
         float cs = c.smagorinsky_c;
         if (!std::isfinite(cs) || cs < 0.0f) cs = 0.12f;
         if (cs > 0.3f) cs = 0.3f;
@@ -215,8 +221,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         if ((rc = solid_u8.alloc(NS))) return rc;
         if (s.cfg.use_les)
             for (int b = 0; b < 2; ++b)
-                for (int k = 0; k < 3; ++k)
-                    if ((rc = u[b][k].alloc(NS * sizeof(U)))) return rc;
+                if ((rc = u[b].alloc(NS * VelAos<U>::stride * sizeof(U)))) return rc;
         for (auto& x : fld) if ((rc = x.alloc(NS * sizeof(R)))) return rc;
         for (auto& x : world) if ((rc = x.alloc(NS * sizeof(R)))) return rc;
         if (s.thermal_ready) {
@@ -302,8 +307,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         for (int k = 1; k < 6; ++k) if ((rc = fill<R>(fld[k], n, R(0)))) return rc;
         for (auto& x : world) if ((rc = fill<R>(x, n, R(0)))) return rc;
         if (s.cfg.use_les)
-            for (int b = 0; b < 2; ++b)
-                for (int k = 0; k < 3; ++k) if ((rc = fill<U>(u[b][k], n, U(0)))) return rc;
+            for (int b = 0; b < 2; ++b) if ((rc = fill<U>(u[b], n * VelAos<U>::stride, U(0)))) return rc;
         if (s.thermal_ready) {
             std::vector<float> Th;
             thermal_initial_field(s, Th);
@@ -374,8 +378,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         B.f_in = first ? f[p].p : f[p ^ 1].p;
         B.f_out = f[p].p;
         B.mask = mask.as<uint32_t>();
-        if (s.cfg.use_les)
-            for (int c = 0; c < 3; ++c) { B.u_prev[c] = u[p][c].p; B.u_next[c] = u[p ^ 1][c].p; }
+        if (s.cfg.use_les) { B.u_prev = u[p].p; B.u_next = u[p ^ 1].p; }
         for (int c = 0; c < 6; ++c) B.fld[c] = fld[c].p;
         if (s.thermal_ready) {
             B.g_in = g[p].p; B.g_out = g[p ^ 1].p;
@@ -437,8 +440,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             // reset the fluid to the inflow equilibrium (aeros/src/lbm.cpp:1583-1600)
             const int p = cur & 1;
             hipLaunchKernelGGL((k_reset<R, S, U>), dim3(grid_for(n)), dim3(256), 0, st, f[p].as<S>(),
-                               mask.as<uint32_t>(), n, P, s.cfg.use_les ? u[p][0].as<U>() : nullptr,
-                               s.cfg.use_les ? u[p][1].as<U>() : nullptr, s.cfg.use_les ? u[p][2].as<U>() : nullptr,
+                               mask.as<uint32_t>(), n, P, s.cfg.use_les ? u[p].as<U>() : nullptr,
                                fld[0].as<R>(), fld[1].as<R>(), fld[2].as<R>(), fld[3].as<R>());
             HIP_TRY(hipGetLastError());
             pre = true;
@@ -655,6 +657,16 @@ int build_handle(aeros_lbm* h) {
     char msg[512] = {0};
     int rc = setup_domain(h->user_cfg, h->s, msg, sizeof msg);
     if (rc) return set_err(rc, "%s", msg);
+    // kernels index with 32-bit offsets: 19 n elements (double) or 19 n sizeof(S)
+    // bytes (float, half, bf16) must stay below 2^31 / 2^32
+    {
+        const int prec = h->s.cfg.precision;
+        const int64_t es = prec == AEROS_FP64 ? 8 : (prec == AEROS_FP32 ? 4 : 2);
+        const int64_t lim = prec == AEROS_FP64 ? (((int64_t)1 << 31) / Q - 1) : (((int64_t)1 << 32) / (Q * es) - 1);
+        if (h->s.total > lim)
+            return set_err(AEROS_ERR_TOO_LARGE, "%lld cells exceed the %lld-cell limit of one lattice at this precision",
+                           (long long)h->s.total, (long long)lim);
+    }
     rc = setup_after_mask(h->s, msg, sizeof msg);
     if (rc) return set_err(rc, "%s", msg);
     h->eng.reset(make_engine(h));

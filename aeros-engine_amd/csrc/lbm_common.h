@@ -79,6 +79,7 @@ template <class R> struct StepParams {
     int write_fields;        // write rho/u/tke/strain fields this step
     int dry;                 // re-run of an aborted step: no abort check, no reduction
     R tau0, cs, vel_scale2;  // sanitised tau, Smagorinsky constant, (flowSpeed/U0)^2
+    R inv_tau0, s_minus0;    // 1/tau0 and the TRT s- at tau0 (exactly what the kernel would compute)
     R vd[7];                 // Van Driest table
     R fx, fy, fz;            // body force
     R in_u[3];               // inlet velocity

@@ -47,18 +47,27 @@ template <class R> __device__ __host__ inline R w7_of(int q) { return q == 0 ? R
 // ------------------------------------------------------------ storage
 struct bf16_t { uint16_t bits; };
 
+// Element i of an array through a 32-bit byte offset: with a uniform base the
+// load or store uses the scalar-base + 32-bit vector-offset address form.
+template <class T> __device__ inline const T& at(const T* base, uint32_t i) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+}
+template <class T> __device__ inline T& at(T* base, uint32_t i) {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T));
+}
+
 template <class S, class R> struct Pop;
 template <> struct Pop<float, float> {
-    __device__ static float ld(const float* p, int64_t i, float) { return p[i]; }
-    __device__ static void st(float* p, int64_t i, float v, float) { p[i] = v; }
+    __device__ static float ld(const float* p, uint32_t i, float) { return at(p, i); }
+    __device__ static void st(float* p, uint32_t i, float v, float) { at(p, i) = v; }
 };
 template <> struct Pop<double, double> {
-    __device__ static double ld(const double* p, int64_t i, double) { return p[i]; }
-    __device__ static void st(double* p, int64_t i, double v, double) { p[i] = v; }
+    __device__ static double ld(const double* p, uint32_t i, double) { return at(p, i); }
+    __device__ static void st(double* p, uint32_t i, double v, double) { at(p, i) = v; }
 };
 template <> struct Pop<_Float16, float> {   // deviation from w_q, IEEE half, round to nearest even
-    __device__ static float ld(const _Float16* p, int64_t i, float w) { return w + (float)p[i]; }
-    __device__ static void st(_Float16* p, int64_t i, float v, float w) { p[i] = (_Float16)(v - w); }
+    __device__ static float ld(const _Float16* p, uint32_t i, float w) { return w + (float)at(p, i); }
+    __device__ static void st(_Float16* p, uint32_t i, float v, float w) { at(p, i) = (_Float16)(v - w); }
 };
 __device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ inline uint16_t f_to_bf16(float x) {   // round to nearest even
@@ -68,23 +77,23 @@ __device__ inline uint16_t f_to_bf16(float x) {   // round to nearest even
     return (uint16_t)(u >> 16);
 }
 template <> struct Pop<bf16_t, float> {     // deviation from w_q, bfloat16
-    __device__ static float ld(const bf16_t* p, int64_t i, float w) { return w + bf16_to_f(p[i].bits); }
-    __device__ static void st(bf16_t* p, int64_t i, float v, float w) { p[i].bits = f_to_bf16(v - w); }
+    __device__ static float ld(const bf16_t* p, uint32_t i, float w) { return w + bf16_to_f(at(p, i).bits); }
+    __device__ static void st(bf16_t* p, uint32_t i, float v, float w) { at(p, i).bits = f_to_bf16(v - w); }
 };
 
-template <class U, class R> struct Vel;
-template <> struct Vel<float, float> {
-    __device__ static float ld(const float* p, int64_t i) { return p[i]; }
-    __device__ static void st(float* p, int64_t i, float v) { p[i] = v; }
-};
-template <> struct Vel<double, double> {
-    __device__ static double ld(const double* p, int64_t i) { return p[i]; }
-    __device__ static void st(double* p, int64_t i, double v) { p[i] = v; }
-};
-template <> struct Vel<_Float16, float> {
-    __device__ static float ld(const _Float16* p, int64_t i) { return (float)p[i]; }
-    __device__ static void st(_Float16* p, int64_t i, float v) { p[i] = (_Float16)v; }
-};
+
+// Population q of cell idx in a structure-of-arrays lattice of n cells.  For
+// element types up to 4 bytes the whole lattice is addressed from one base
+// with a 32-bit byte offset (the host guarantees 19 n sizeof(S) < 2^32);
+// doubles use a per-direction base.
+template <class S, class R> __device__ inline R ldq(const S* f, uint32_t n, int q, uint32_t idx, R w) {
+    if constexpr (sizeof(S) <= 4) return Pop<S, R>::ld(f, (uint32_t)q * n + idx, w);
+    else return Pop<S, R>::ld(f + (size_t)q * n, idx, w);
+}
+template <class S, class R> __device__ inline void stq(S* f, uint32_t n, int q, uint32_t idx, R v, R w) {
+    if constexpr (sizeof(S) <= 4) Pop<S, R>::st(f, (uint32_t)q * n + idx, v, w);
+    else Pop<S, R>::st(f + (size_t)q * n, idx, v, w);
+}
 
 template <class R> __device__ inline bool fin(R x) { return __builtin_isfinite(x); }
 
@@ -106,6 +115,17 @@ template <class R, int q> __device__ inline R feq_k(R rho, R ux, R uy, R uz, R u
     const R cu = cdotu<R, q>(ux, uy, uz);
     return w_of<R>(q) * rho * (R(1) + cu * R(3) + cu * cu * R(4.5) - usqr * R(1.5));
 }
+// Same value as feq_k for a runtime direction (used only on rare fix-up paths).
+template <class R> __device__ inline R feq_rt(int q, R rho, R ux, R uy, R uz, R usqr) {
+    const int a = cx_of(q), b = cy_of(q), c = cz_of(q);
+    R s = R(0);
+    bool have = false;
+    if (a != 0) { s = a > 0 ? ux : -ux; have = true; }
+    if (b != 0) { const R t = b > 0 ? uy : -uy; s = have ? s + t : t; have = true; }
+    if (c != 0) { const R t = c > 0 ? uz : -uz; s = have ? s + t : t; have = true; }
+    return w_of<R>(q) * rho * (R(1) + s * R(3) + s * s * R(4.5) - usqr * R(1.5));
+}
+
 // computeEquilibriumFast (aeros/src/lbm.cpp:146-184), guarded form
 template <class R> __device__ inline R feq_guarded(int q, R rho, R ux, R uy, R uz, R usqr) {
     if (!fin(rho)) rho = R(1);
@@ -253,8 +273,8 @@ struct StepBuffers {
     const void* f_in;
     void* f_out;
     const uint32_t* mask;
-    const void* u_prev[3];     // LES stencil input (previous step's velocity)
-    void* u_next[3];
+    const void* u_prev;        // Smagorinsky stencil input: previous step's velocity, AoS (x,y,z[,pad])
+    void* u_next;
     void* fld[6];              // rho, ux, uy, uz, tke, strain (type R)
     const void* g_in;          // thermal populations (type R)
     void* g_out;
@@ -265,6 +285,11 @@ struct StepBuffers {
     Status* status;
 };
 
+// Velocity stencil storage: 3 components, padded to 4 for half so that one
+// 8-byte load fetches a neighbour.
+template <class U> struct VelAos { static constexpr int stride = 3; };
+template <> struct VelAos<_Float16> { static constexpr int stride = 4; };
+
 template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
 __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) {
     __shared__ RedLds L;
@@ -274,52 +299,57 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
 
     const S* fin_ = static_cast<const S*>(B.f_in);
     S* fout = static_cast<S*>(B.f_out);
-    const int64_t n = P.n;
+    const uint32_t n = (uint32_t)P.n;
     const int nx = P.nx, ny = P.ny, nz = P.nz;
-    const int64_t sy = nx, sz = (int64_t)nx * ny;
+    const uint32_t sy = (uint32_t)nx, sz = (uint32_t)nx * (uint32_t)ny;
     const int tid = threadIdx.x;
+    constexpr int US = VelAos<U>::stride;
 
     Partial acc{0.0, 0.0, 0.0, 0.0f, 0, 0, 0, 0, 0, 0};
+    R maxu2 = R(0);
 
-    const int64_t stride = (int64_t)gridDim.x * BLOCK;
-    for (int64_t cell = (int64_t)blockIdx.x * BLOCK + tid; cell < n; cell += stride) {
-        const uint32_t m = B.mask[cell];
+    const uint32_t stride = gridDim.x * BLOCK;
+    for (uint32_t cell = blockIdx.x * BLOCK + tid; cell < n; cell += stride) {
+        const uint32_t m = at(B.mask, cell);
         if (m & M_SOLID) continue;   // solid/ground: fields are constant (rho 1, u 0), never stored
 
         // ---------------- pull S(x, q) from the previous post-collision state
         R f[Q];
         if (P.first) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) f[q] = Pop<S, R>::ld(fin_, q * n + cell, w_of<R>(q));
+            for (int q = 0; q < Q; ++q) f[q] = ldq<S, R>(fin_, n, q, cell, w_of<R>(q));
+        } else if (!(m & M_SHELL)) {
+            // interior: the source of q is cell - c_q; a uniform shifted base per direction
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int off = cx_of(q) + cy_of(q) * nx + cz_of(q) * (int)sz;
+                f[q] = ldq<S, R>(fin_, n, q, cell - (uint32_t)off, w_of<R>(q));
+            }
+            if (m & M_LINKS) {   // half-way bounce-back: the cell's own opposite population
+#pragma unroll
+                for (int q = 1; q < Q; ++q)
+                    if ((m >> q) & 1u) f[q] = ldq<S, R>(fin_, n, opp_of(q), cell, w_of<R>(opp_of(q)));
+            }
         } else if (m & M_PULL_INLET) {
 #pragma unroll
             for (int q = 0; q < Q; ++q) f[q] = P.feq_in[q];
         } else if (m & M_PULL_OUTLET) {
             // f_out + c (f_in - f_out) + 0.15 c (f_in - f_in2) on the planes N-1, N-2, N-3
             // (aeros/src/lbm.cpp:1465-1544)
-            const int64_t ci = cell - P.out_d1, ci2 = cell - P.out_d2;
+            const uint32_t ci = (uint32_t)((int64_t)cell - P.out_d1), ci2 = (uint32_t)((int64_t)cell - P.out_d2);
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const R w = w_of<R>(q);
-                const R fo = Pop<S, R>::ld(fin_, q * n + cell, w);
-                const R fi = Pop<S, R>::ld(fin_, q * n + ci, w);
-                const R fi2 = Pop<S, R>::ld(fin_, q * n + ci2, w);
+                const R fo = ldq<S, R>(fin_, n, q, cell, w);
+                const R fi = ldq<S, R>(fin_, n, q, ci, w);
+                const R fi2 = ldq<S, R>(fin_, n, q, ci2, w);
                 f[q] = fo + P.conv_c * (fi - fo) + P.conv_k2 * (fi - fi2);
-            }
-        } else if (!(m & M_SHELL)) {
-            // interior: every source is in the domain
-#pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const bool bb = (m >> q) & 1u;
-                const int64_t src = bb ? cell : cell - (cx_of(q) + cy_of(q) * sy + cz_of(q) * sz);
-                const int dq = bb ? opp_of(q) : q;
-                f[q] = Pop<S, R>::ld(fin_, dq * n + src, w_of<R>(dq));
             }
         } else {
             // domain shell: periodic wrap, out-of-domain sources copy the
             // cell's own population (aeros/src/lbm.cpp:1362-1409)
-            const int x = (int)(cell % nx);
-            const int y = (int)((cell / nx) % ny);
+            const int x = (int)(cell % (uint32_t)nx);
+            const int y = (int)((cell / (uint32_t)nx) % (uint32_t)ny);
             const int z = (int)(cell / sz);
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
@@ -328,26 +358,42 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
                 if (P.periodic_z) { if (szz < 0) szz += nz; else if (szz >= nz) szz -= nz; }
                 const bool out = sx < 0 || sx >= nx || syy < 0 || syy >= ny || szz < 0 || szz >= nz;
                 const bool bb = !out && ((m >> q) & 1u);
-                const int64_t src = (out || bb) ? cell : ((int64_t)szz * ny + syy) * nx + sx;
+                const uint32_t src = (out || bb) ? cell : ((uint32_t)szz * (uint32_t)ny + (uint32_t)syy) * (uint32_t)nx + (uint32_t)sx;
                 const int dq = bb ? opp_of(q) : q;
-                f[q] = Pop<S, R>::ld(fin_, dq * n + src, w_of<R>(dq));
+                f[q] = ldq<S, R>(fin_, n, dq, src, w_of<R>(dq));
             }
         }
 
         // ---------------- moments with sanitising (aeros/src/lbm.cpp:1061-1089)
+        // Every population in [-1, 10] (which also excludes NaN/Inf) is the
+        // common case; only otherwise are populations replaced and counted.
+        bool clean = true;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) clean = clean && (f[q] >= R(-1)) && (f[q] <= R(10));
         R rho = R(0), ux = R(0), uy = R(0), uz = R(0);
         int nanc = 0, badc = 0;
+        if (clean) {
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            R fi = f[q];
-            if (!fin(fi)) { fi = w_of<R>(q); nanc++; badc++; }
-            if (fi < R(-1) || fi > R(10)) { fi = w_of<R>(q); badc++; }
-            // rho += f; u += f c with c in {-1,0,1}: the products by 0 only add
-            // signed zeros, so they are left out
-            rho += fi;
-            if (cx_of(q) > 0) ux += fi; else if (cx_of(q) < 0) ux -= fi;
-            if (cy_of(q) > 0) uy += fi; else if (cy_of(q) < 0) uy -= fi;
-            if (cz_of(q) > 0) uz += fi; else if (cz_of(q) < 0) uz -= fi;
+            for (int q = 0; q < Q; ++q) {
+                // rho += f; u += f c with c in {-1,0,1}: the products by 0 only add
+                // signed zeros, so they are left out
+                const R fi = f[q];
+                rho += fi;
+                if (cx_of(q) > 0) ux += fi; else if (cx_of(q) < 0) ux -= fi;
+                if (cy_of(q) > 0) uy += fi; else if (cy_of(q) < 0) uy -= fi;
+                if (cz_of(q) > 0) uz += fi; else if (cz_of(q) < 0) uz -= fi;
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                R fi = f[q];
+                if (!fin(fi)) { fi = w_of<R>(q); nanc++; badc++; }
+                if (fi < R(-1) || fi > R(10)) { fi = w_of<R>(q); badc++; }
+                rho += fi;
+                if (cx_of(q) > 0) ux += fi; else if (cx_of(q) < 0) ux -= fi;
+                if (cy_of(q) > 0) uy += fi; else if (cy_of(q) < 0) uy -= fi;
+                if (cz_of(q) > 0) uz += fi; else if (cz_of(q) < 0) uz -= fi;
+            }
         }
         if (!fin(rho)) { acc.rho_nf++; nanc++; rho = R(1); }
         else if (rho < R(0.5) || rho > R(1.5)) { acc.rho_out++; nanc++; rho = R(1); }
@@ -368,19 +414,20 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
         R tau_eff = P.tau0, smag = R(0);
         if (LES && !inlet && !outlet) {
             if (m & M_STRAIN) {
-                const U* ux0 = static_cast<const U*>(B.u_prev[0]);
-                const U* uy0 = static_cast<const U*>(B.u_prev[1]);
-                const U* uz0 = static_cast<const U*>(B.u_prev[2]);
+                const U* u0 = static_cast<const U*>(B.u_prev);
+                auto ld3 = [&](uint32_t c, R& a, R& b, R& d) {
+                    const U* p = &at(u0, c * (uint32_t)US);
+                    a = (R)p[0]; b = (R)p[1]; d = (R)p[2];
+                };
+                R xpx, xpy, xpz, xmx, xmy, xmz, ypx, ypy, ypz, ymx, ymy, ymz, zpx, zpy, zpz, zmx, zmy, zmz;
+                ld3(cell + 1, xpx, xpy, xpz); ld3(cell - 1, xmx, xmy, xmz);
+                ld3(cell + sy, ypx, ypy, ypz); ld3(cell - sy, ymx, ymy, ymz);
+                ld3(cell + sz, zpx, zpy, zpz); ld3(cell - sz, zmx, zmy, zmz);
                 const R h = R(0.5);
-                const R dux_dx = (Vel<U, R>::ld(ux0, cell + 1) - Vel<U, R>::ld(ux0, cell - 1)) * h;
-                const R duy_dy = (Vel<U, R>::ld(uy0, cell + sy) - Vel<U, R>::ld(uy0, cell - sy)) * h;
-                const R duz_dz = (Vel<U, R>::ld(uz0, cell + sz) - Vel<U, R>::ld(uz0, cell - sz)) * h;
-                const R dux_dy = (Vel<U, R>::ld(ux0, cell + sy) - Vel<U, R>::ld(ux0, cell - sy)) * h;
-                const R dux_dz = (Vel<U, R>::ld(ux0, cell + sz) - Vel<U, R>::ld(ux0, cell - sz)) * h;
-                const R duy_dx = (Vel<U, R>::ld(uy0, cell + 1) - Vel<U, R>::ld(uy0, cell - 1)) * h;
-                const R duy_dz = (Vel<U, R>::ld(uy0, cell + sz) - Vel<U, R>::ld(uy0, cell - sz)) * h;
-                const R duz_dx = (Vel<U, R>::ld(uz0, cell + 1) - Vel<U, R>::ld(uz0, cell - 1)) * h;
-                const R duz_dy = (Vel<U, R>::ld(uz0, cell + sy) - Vel<U, R>::ld(uz0, cell - sy)) * h;
+                const R dux_dx = (xpx - xmx) * h, duy_dy = (ypy - ymy) * h, duz_dz = (zpz - zmz) * h;
+                const R dux_dy = (ypx - ymx) * h, dux_dz = (zpx - zmx) * h;
+                const R duy_dx = (xpy - xmy) * h, duy_dz = (zpy - zmy) * h;
+                const R duz_dx = (xpz - xmz) * h, duz_dy = (ypz - ymz) * h;
                 const R sxy = h * (dux_dy + duy_dx), sxz = h * (dux_dz + duz_dx), syz = h * (duy_dz + duz_dy);
                 const R s2 = dux_dx * dux_dx + duy_dy * duy_dy + duz_dz * duz_dz + R(2) * (sxy * sxy + sxz * sxz + syz * syz);
                 smag = __builtin_sqrt(R(2) * s2);
@@ -403,12 +450,13 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
             ux *= s; uy *= s; uz *= s;
             usqr = ux * ux + uy * uy + uz * uz;
         }
-        const R inv_tau = R(1) / tau_eff;
+        const bool base_tau = tau_eff == P.tau0;
+        const R inv_tau = base_tau ? P.inv_tau0 : R(1) / tau_eff;
 
         // ---------------- Guo forcing (aeros/src/lbm.cpp:1161-1176)
         R Fx = R(0), Fy = R(0), Fz = R(0);
         if (FORCE) { Fx = P.fx; Fy = P.fy; Fz = P.fz; }
-        if (THERM) Fy += static_cast<const R*>(B.b_in)[cell];
+        if (THERM) Fy += at(static_cast<const R*>(B.b_in), cell);
         const bool has_force = (FORCE || THERM) && (Fx != R(0) || Fy != R(0) || Fz != R(0));
         R ex = ux, ey = uy, ez = uz, es = usqr;
         if (has_force) {
@@ -417,22 +465,22 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
             es = ex * ex + ey * ey + ez * ez;
         }
         R s_minus = inv_tau;
-        if (P.trt) {   // magic parameter 3/16 (aeros/src/lbm.cpp:1196-1202)
+        if (P.trt && base_tau) s_minus = P.s_minus0;
+        else if (P.trt) {   // magic parameter 3/16 (aeros/src/lbm.cpp:1196-1202)
             const R mm = inv_tau - R(0.5);
             const R den = (__builtin_fabs(mm) > R(1e-6)) ? (R(0.1875) / mm + R(0.5)) : R(1);
             s_minus = R(1) / (den > R(0.05) ? den : R(0.05));
         }
 
         // ---------------- pairwise TRT/BGK relaxation (aeros/src/lbm.cpp:1204-1264)
-        int negc = 0;
+        // Inputs are finite on the clean path, so the relaxation's "non-finite
+        // input -> equilibrium" replacement only runs otherwise.
         {
             const R feq0 = feq_k<R, 0>(rho, ex, ey, ez, es);
             R fi = f[0];
-            if (!fin(fi)) fi = feq0;
+            if (!clean && !fin(fi)) fi = feq0;
             R fn = fi - (fi - feq0) * inv_tau;
             if (has_force) fn = fn + (R(1) - R(0.5) * inv_tau) * guo<R, 0>(ex, ey, ez, Fx, Fy, Fz);
-            if (!fin(fn)) fn = feq0;
-            if (fn < R(0)) negc++;
             f[0] = fn;
         }
         const R kp = R(1) - R(0.5) * inv_tau, km = R(1) - R(0.5) * s_minus;
@@ -442,8 +490,10 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
             const R feq = feq_k<R, qi>(rho, ex, ey, ez, es);                                        \
             const R feqo = feq_k<R, qo>(rho, ex, ey, ez, es);                                       \
             R fi = f[qi], fo = f[qo];                                                               \
-            if (!fin(fi)) fi = feq;                                                                 \
-            if (!fin(fo)) fo = feqo;                                                                \
+            if (!clean) {                                                                           \
+                if (!fin(fi)) fi = feq;                                                             \
+                if (!fin(fo)) fo = feqo;                                                            \
+            }                                                                                       \
             const R fp = R(0.5) * (fi + fo), fm = R(0.5) * (fi - fo);                               \
             const R ep = R(0.5) * (feq + feqo), em = R(0.5) * (feq - feqo);                         \
             const R fpn = fp - inv_tau * (fp - ep);                                                 \
@@ -459,17 +509,26 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
                 ni = fpn + fmn;                                                                     \
                 no = fpn - fmn;                                                                     \
             }                                                                                       \
-            if (!fin(ni)) ni = feq;                                                                 \
-            if (!fin(no)) no = feqo;                                                                \
-            negc += (ni < R(0)) + (no < R(0));                                                      \
             f[qi] = ni; f[qo] = no;                                                                 \
         }
         AEROS_PAIR(1) AEROS_PAIR(3) AEROS_PAIR(5) AEROS_PAIR(7) AEROS_PAIR(9)
         AEROS_PAIR(11) AEROS_PAIR(13) AEROS_PAIR(15) AEROS_PAIR(17)
 #undef AEROS_PAIR
+        // non-finite results fall back to equilibrium; count negatives
+        bool fin_all = true;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) fin_all = fin_all && fin(f[q]);
+        if (!fin_all) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (!fin(f[q])) f[q] = feq_rt<R>(q, rho, ex, ey, ez, es);
+        }
+        int negc = 0;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) negc += f[q] < R(0);
 
 #pragma unroll
-        for (int q = 0; q < Q; ++q) Pop<S, R>::st(fout, q * n + cell, f[q], w_of<R>(q));
+        for (int q = 0; q < Q; ++q) stq<S, R>(fout, n, q, cell, f[q], w_of<R>(q));
 
         // ---------------- velocity and diagnostics (aeros/src/lbm.cpp:1266-1305)
         R vx = ux, vy = uy, vz = uz;   // the stored velocity includes F/(2 rho)
@@ -478,66 +537,79 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
             vx = ux + Fx * hi; vy = uy + Fy * hi; vz = uz + Fz * hi;
         }
         if (LES) {
-            Vel<U, R>::st(static_cast<U*>(B.u_next[0]), cell, vx);
-            Vel<U, R>::st(static_cast<U*>(B.u_next[1]), cell, vy);
-            Vel<U, R>::st(static_cast<U*>(B.u_next[2]), cell, vz);
+            U* un = &at(static_cast<U*>(B.u_next), cell * (uint32_t)US);
+            un[0] = (U)vx; un[1] = (U)vy; un[2] = (U)vz;
+            if (US == 4) un[3] = (U)0;
         }
         const R tl = (P.cs * smag) * (P.cs * smag);
         R tk = tl * P.vel_scale2;
         if (!fin(tk) || tk < R(0)) tk = R(0);
         if (tk > R(1.0e6)) tk = R(1.0e6);
         if (P.write_fields) {
-            static_cast<R*>(B.fld[0])[cell] = rho;
-            static_cast<R*>(B.fld[1])[cell] = vx;
-            static_cast<R*>(B.fld[2])[cell] = vy;
-            static_cast<R*>(B.fld[3])[cell] = vz;
-            static_cast<R*>(B.fld[4])[cell] = tk;
-            static_cast<R*>(B.fld[5])[cell] = smag;
+            at(static_cast<R*>(B.fld[0]), cell) = rho;
+            at(static_cast<R*>(B.fld[1]), cell) = vx;
+            at(static_cast<R*>(B.fld[2]), cell) = vy;
+            at(static_cast<R*>(B.fld[3]), cell) = vz;
+            at(static_cast<R*>(B.fld[4]), cell) = tk;
+            at(static_cast<R*>(B.fld[5]), cell) = smag;
         }
-        const float vmag = (float)__builtin_sqrt(usqr);
-        acc.maxv = fmaxf(acc.maxv, vmag);
+        maxu2 = usqr > maxu2 ? usqr : maxu2;   // max |u|^2; the sqrt is taken once per thread
         acc.rho += (double)rho;
         acc.ke += (double)usqr;
         acc.tke += (double)tk;
         acc.nan += nanc; acc.bad += badc; acc.neg += negc;
 
         // ---------------- D3Q7 temperature: pull, then BGK with the new velocity
-        // (aeros/src/thermal_lbm.cpp:205-336)
+        // (aeros/src/thermal_lbm.cpp:205-336).  x and z always wrap for this
+        // lattice; a solid source bounces back, anti-bounce-back with the wall
+        // temperature on the rows y <= 0 and y >= Ny-1.
         if (THERM) {
             const R* g = static_cast<const R*>(B.g_in);
             R* go = static_cast<R*>(B.g_out);
-            const int x = (int)(cell % nx);
-            const int y = (int)((cell / nx) % ny);
-            const int z = (int)(cell / sz);
             R gi[Q7];
+            if (!(m & M_SHELL)) {
 #pragma unroll
-            for (int i = 0; i < Q7; ++i) {
-                int sx = x - cx_of(i), syy = y - cy_of(i), szz = z - cz_of(i);
-                // x and z always periodic for the temperature lattice
-                if (sx < 0) sx = nx - 1; else if (sx >= nx) sx = 0;
-                if (szz < 0) szz = nz - 1; else if (szz >= nz) szz = 0;
-                const bool outy = syy < 0 || syy >= ny;
-                bool bounce = outy;
-                int wy = syy < 0 ? 0 : ny - 1;
-                if (!outy) {
-                    const int64_t src = ((int64_t)szz * ny + syy) * nx + sx;
-                    bounce = (B.mask[src] & M_SOLID) != 0;
-                    wy = syy;
+                for (int i = 0; i < Q7; ++i) {
+                    const int off = cx_of(i) + cy_of(i) * nx + cz_of(i) * (int)sz;
+                    if ((m >> i) & 1u) {
+                        const R go_ = at(g + (size_t)opp_of(i) * n, cell);
+                        const int wy = (int)((cell / (uint32_t)nx) % (uint32_t)ny) - cy_of(i);
+                        gi[i] = (P.t_fixed && (wy <= 0 || wy >= ny - 1))
+                                    ? (-go_ + R(2) * w7_of<R>(i) * (wy <= 0 ? P.t_hot : P.t_cold)) : go_;
+                    } else {
+                        gi[i] = at(g + (size_t)i * n - off, cell);
+                    }
                 }
-                if (bounce) {
-                    const R go_ = g[opp_of(i) * n + cell];
-                    // anti-bounce-back with the wall temperature (aeros/src/thermal_lbm.cpp:60-65, 286-293)
-                    if (P.t_fixed && (wy <= 0 || wy >= ny - 1))
-                        gi[i] = -go_ + R(2) * w7_of<R>(i) * (wy <= 0 ? P.t_hot : P.t_cold);
-                    else
-                        gi[i] = go_;
-                } else {
-                    gi[i] = g[i * n + (((int64_t)szz * ny + syy) * nx + sx)];
+            } else {
+                const int x = (int)(cell % (uint32_t)nx);
+                const int y = (int)((cell / (uint32_t)nx) % (uint32_t)ny);
+                const int z = (int)(cell / sz);
+#pragma unroll
+                for (int i = 0; i < Q7; ++i) {
+                    int sx = x - cx_of(i), syy = y - cy_of(i), szz = z - cz_of(i);
+                    if (sx < 0) sx = nx - 1; else if (sx >= nx) sx = 0;
+                    if (szz < 0) szz = nz - 1; else if (szz >= nz) szz = 0;
+                    const bool outy = syy < 0 || syy >= ny;
+                    bool bounce = outy;
+                    int wy = syy < 0 ? 0 : ny - 1;
+                    uint32_t src = 0;
+                    if (!outy) {
+                        src = ((uint32_t)szz * (uint32_t)ny + (uint32_t)syy) * (uint32_t)nx + (uint32_t)sx;
+                        bounce = (at(B.mask, src) & M_SOLID) != 0;
+                        wy = syy;
+                    }
+                    if (bounce) {
+                        const R go_ = at(g + (size_t)opp_of(i) * n, cell);
+                        gi[i] = (P.t_fixed && (wy <= 0 || wy >= ny - 1))
+                                    ? (-go_ + R(2) * w7_of<R>(i) * (wy <= 0 ? P.t_hot : P.t_cold)) : go_;
+                    } else {
+                        gi[i] = at(g + (size_t)i * n, src);
+                    }
                 }
             }
             R T = R(0);
 #pragma unroll
-            for (int i = 0; i < Q7; ++i) T = i == 0 ? gi[0] : T + gi[i];
+            for (int i = 0; i < Q7; ++i) T += gi[i];
             if (!fin(T)) T = P.t_ref;
             T = (P.t_lo > T ? P.t_lo : T);
             T = (P.t_hi < T ? P.t_hi : T);
@@ -545,15 +617,18 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
             for (int i = 0; i < Q7; ++i) {
                 const R cu = (R)cx_of(i) * vx + (R)cy_of(i) * vy + (R)cz_of(i) * vz;
                 const R e = w7_of<R>(i) * T * (R(1) + R(4) * cu);
-                go[i * n + cell] = gi[i] - (gi[i] - e) * P.t_inv_tau;
+                at(go + (size_t)i * n, cell) = gi[i] - (gi[i] - e) * P.t_inv_tau;
             }
-            static_cast<R*>(B.t_out)[cell] = T;
-            static_cast<R*>(B.b_out)[cell] = P.t_g * (T - P.t_ref);
+            at(static_cast<R*>(B.t_out), cell) = T;
+            at(static_cast<R*>(B.b_out), cell) = P.t_g * (T - P.t_ref);
         }
     }
 
     // ---------------- fixed-order reduction and end-of-step logic
     if (P.dry) return;
+    // max sqrt(usqr) == sqrt(max usqr): correctly rounded sqrt and the cast
+    // to float are monotone, so the block maximum is unchanged.
+    acc.maxv = (float)__builtin_sqrt(maxu2);
     block_reduce(L, tid, acc);
     if (tid == 0) {
         B.parts[blockIdx.x] = acc;
@@ -586,20 +661,24 @@ __global__ void k_init(S* fa, S* fb, const uint32_t* mask, const float* u3, int6
             a *= s; b *= s; d *= s; us = a * a + b * b + d * d;
         }
         for (int q = 0; q < Q; ++q) {
-            Pop<S, R>::st(fa, q * n + c, feq_guarded<R>(q, R(1), a, b, d, us), w_of<R>(q));
-            Pop<S, R>::st(fb, q * n + c, R(0), w_of<R>(q));
+            Pop<S, R>::st(fa + (size_t)q * n, (uint32_t)c, feq_guarded<R>(q, R(1), a, b, d, us), w_of<R>(q));
+            Pop<S, R>::st(fb + (size_t)q * n, (uint32_t)c, R(0), w_of<R>(q));
         }
     }
 }
 
 // Divergence reset of the fluid cells (aeros/src/lbm.cpp:1583-1600).
 template <class R, class S, class U>
-__global__ void k_reset(S* f, const uint32_t* mask, int64_t n, StepParams<R> P, U* u0, U* u1, U* u2, R* rho,
-                        R* fux, R* fuy, R* fuz) {
+__global__ void k_reset(S* f, const uint32_t* mask, int64_t n, StepParams<R> P, U* uaos, R* rho, R* fux, R* fuy,
+                        R* fuz) {
+    constexpr int US = VelAos<U>::stride;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
         if (mask[c] & M_SOLID) continue;
-        for (int q = 0; q < Q; ++q) Pop<S, R>::st(f, q * n + c, P.feq_in[q], w_of<R>(q));
-        if (u0) { Vel<U, R>::st(u0, c, P.in_u[0]); Vel<U, R>::st(u1, c, P.in_u[1]); Vel<U, R>::st(u2, c, P.in_u[2]); }
+        for (int q = 0; q < Q; ++q) Pop<S, R>::st(f + (size_t)q * n, (uint32_t)c, P.feq_in[q], w_of<R>(q));
+        if (uaos) {
+            U* u = uaos + (size_t)c * US;
+            u[0] = (U)P.in_u[0]; u[1] = (U)P.in_u[1]; u[2] = (U)P.in_u[2];
+        }
         rho[c] = R(1); fux[c] = P.in_u[0]; fuy[c] = P.in_u[1]; fuz[c] = P.in_u[2];
     }
 }
@@ -656,7 +735,7 @@ __global__ void k_pull_only(const S* fin_, const S* fsolid, R* out, const uint32
     for (int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; cell < n; cell += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t m = mask[cell];
         if (m & M_SOLID) {
-            for (int q = 0; q < Q; ++q) out[q * n + cell] = Pop<S, R>::ld(fsolid, q * n + cell, w_of<R>(q));
+            for (int q = 0; q < Q; ++q) out[q * n + cell] = Pop<S, R>::ld(fsolid + (size_t)q * n, (uint32_t)cell, w_of<R>(q));
             continue;
         }
         if (m & M_PULL_INLET) {
@@ -667,8 +746,9 @@ __global__ void k_pull_only(const S* fin_, const S* fsolid, R* out, const uint32
             const int64_t ci = cell - P.out_d1, ci2 = cell - P.out_d2;
             for (int q = 0; q < Q; ++q) {
                 const R w = w_of<R>(q);
-                const R fo = Pop<S, R>::ld(fin_, q * n + cell, w), fi = Pop<S, R>::ld(fin_, q * n + ci, w);
-                const R fi2 = Pop<S, R>::ld(fin_, q * n + ci2, w);
+                const S* b = fin_ + (size_t)q * n;
+                const R fo = Pop<S, R>::ld(b, (uint32_t)cell, w), fi = Pop<S, R>::ld(b, (uint32_t)ci, w);
+                const R fi2 = Pop<S, R>::ld(b, (uint32_t)ci2, w);
                 out[q * n + cell] = fo + P.conv_c * (fi - fo) + P.conv_k2 * (fi - fi2);
             }
             continue;
@@ -682,7 +762,7 @@ __global__ void k_pull_only(const S* fin_, const S* fsolid, R* out, const uint32
             const bool bb = !out_ && ((m >> q) & 1u);
             const int64_t src = (out_ || bb) ? cell : ((int64_t)szz * ny + syy) * nx + sx;
             const int dq = bb ? opp_of(q) : q;
-            out[q * n + cell] = Pop<S, R>::ld(fin_, dq * n + src, w_of<R>(dq));
+            out[q * n + cell] = Pop<S, R>::ld(fin_ + (size_t)dq * n, (uint32_t)src, w_of<R>(dq));
         }
         (void)sy;
     }
@@ -692,7 +772,7 @@ __global__ void k_pull_only(const S* fin_, const S* fsolid, R* out, const uint32
 template <class R, class S>
 __global__ void k_widen(const S* in, R* out, int64_t n) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
-        for (int q = 0; q < Q; ++q) out[q * n + c] = Pop<S, R>::ld(in, q * n + c, w_of<R>(q));
+        for (int q = 0; q < Q; ++q) out[q * n + c] = Pop<S, R>::ld(in + (size_t)q * n, (uint32_t)c, w_of<R>(q));
 }
 
 // thermalUpload / thermalResetField populations: g = g_eq(T, 0) = w T.
