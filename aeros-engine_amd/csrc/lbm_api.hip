@@ -208,6 +208,13 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         P.fluid_cells = s.fluid;
         P.nan_limit = n / 10;
         P.conv_threshold = c.convergence_threshold;
+        {   // tau0 is in [0.51, 1.5] and |u| <= 0.3 after the clamps; with a total force
+            // below 1e3 every relaxed population stays below ~1e12
+            const double F = std::fabs((double)c.body_force[0]) + std::fabs((double)c.body_force[1]) +
+                             std::fabs((double)c.body_force[2]) +
+                             (P.thermal ? std::fabs((double)P.t_g) * ((double)P.t_hi - (double)P.t_lo) : 0.0);
+            P.bounded = std::isfinite(F) && F <= 1e3 && std::isfinite((double)P.t_ref) ? 1 : 0;
+        }
         return P;
     }
 

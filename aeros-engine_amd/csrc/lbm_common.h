@@ -95,6 +95,10 @@ template <class R> struct StepParams {
     int64_t nan_limit;       // total / 10
     float vel_scale;         // flowSpeed / U0 (clamped), for max |u| world
     float conv_threshold;
+    // 1 when tau, the body force and the buoyancy bound every collision
+    // output of populations in [-1, 10] far below FLT_MAX, so that the
+    // non-finite fix-up can be skipped for such cells
+    int bounded;
 };
 
 }  // namespace aeros

@@ -81,11 +81,10 @@ template <> struct Pop<bf16_t, float> {     // deviation from w_q, bfloat16
     __device__ static void st(bf16_t* p, uint32_t i, float v, float w) { at(p, i).bits = f_to_bf16(v - w); }
 };
 
-
-// Population q of cell idx in a structure-of-arrays lattice of n cells.  For
-// element types up to 4 bytes the whole lattice is addressed from one base
-// with a 32-bit byte offset (the host guarantees 19 n sizeof(S) < 2^32);
-// doubles use a per-direction base.
+// Population q of cell idx in a structure-of-arrays lattice of n cells, by
+// runtime (per-lane) plane index.  For element types up to 4 bytes the whole
+// lattice is addressed from one base with a 32-bit byte offset (the host
+// guarantees 19 n sizeof(S) < 2^32); doubles use a per-direction base.
 template <class S, class R> __device__ inline R ldq(const S* f, uint32_t n, int q, uint32_t idx, R w) {
     if constexpr (sizeof(S) <= 4) return Pop<S, R>::ld(f, (uint32_t)q * n + idx, w);
     else return Pop<S, R>::ld(f + (size_t)q * n, idx, w);
@@ -94,6 +93,24 @@ template <class S, class R> __device__ inline void stq(S* f, uint32_t n, int q, 
     if constexpr (sizeof(S) <= 4) Pop<S, R>::st(f, (uint32_t)q * n + idx, v, w);
     else Pop<S, R>::st(f + (size_t)q * n, idx, v, w);
 }
+
+// A lattice seen by the step kernel: direction planes of n cells, element
+// plane + idx with plane = q n (less the pull shift) >= 0.  Element types up
+// to 4 bytes use one base and a 32-bit byte offset (the host guarantees
+// 19 n sizeof(S) < 2^32), which keeps the 19 directions off the scalar
+// register file; doubles use a per-direction base.
+template <class S> struct Lat {
+    S* p;
+    __device__ Lat(const void* base) : p(static_cast<S*>(const_cast<void*>(base))) {}
+    template <class R> __device__ R ld(uint32_t idx, uint32_t plane, R w) const {
+        if constexpr (sizeof(S) <= 4) return Pop<S, R>::ld(p, plane + idx, w);
+        else return Pop<S, R>::ld(p + plane, idx, w);
+    }
+    template <class R> __device__ void st(uint32_t idx, uint32_t plane, R v, R w) const {
+        if constexpr (sizeof(S) <= 4) Pop<S, R>::st(p, plane + idx, v, w);
+        else Pop<S, R>::st(p + plane, idx, v, w);
+    }
+};
 
 template <class R> __device__ inline bool fin(R x) { return __builtin_isfinite(x); }
 
@@ -298,8 +315,8 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
     if (!P.dry && st->abort) return;   // an earlier step of this call broke the loop
 
     const S* fin_ = static_cast<const S*>(B.f_in);
-    S* fout = static_cast<S*>(B.f_out);
     const uint32_t n = (uint32_t)P.n;
+    const Lat<S> lin(B.f_in), lout(B.f_out);
     const int nx = P.nx, ny = P.ny, nz = P.nz;
     const uint32_t sy = (uint32_t)nx, sz = (uint32_t)nx * (uint32_t)ny;
     const int tid = threadIdx.x;
@@ -308,27 +325,67 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
     Partial acc{0.0, 0.0, 0.0, 0.0f, 0, 0, 0, 0, 0, 0};
     R maxu2 = R(0);
 
+    // Each iteration issues the next cell's mask load first, then the
+    // Smagorinsky stencil and the populations back to back, so a cell costs
+    // one memory round trip.
     const uint32_t stride = gridDim.x * BLOCK;
-    for (uint32_t cell = blockIdx.x * BLOCK + tid; cell < n; cell += stride) {
-        const uint32_t m = at(B.mask, cell);
+    uint32_t cell = blockIdx.x * BLOCK + tid;
+    uint32_t m_next = cell < n ? at(B.mask, cell) : 0u;
+    for (; cell < n; cell += stride) {
+        const uint32_t m = m_next;
+        if (cell + stride < n) m_next = at(B.mask, cell + stride);
         if (m & M_SOLID) continue;   // solid/ground: fields are constant (rho 1, u 0), never stored
+        const bool inlet = m & M_INLET, outlet = m & M_OUTLET;
+
+        // ---------------- previous-step velocity around the cell for the
+        // Smagorinsky strain (aeros/src/lbm.cpp:979-1036).  With 4- and 8-byte
+        // storage these loads join the population loads, so a cell costs one
+        // memory round trip; with 16-bit storage they follow the moments,
+        // which holds fewer registers and keeps one more wave per SIMD, worth
+        // more there than the round trip.
+        constexpr bool EARLY_U = sizeof(S) >= 4;
+        const bool strain = LES && !inlet && !outlet && (m & M_STRAIN);
+        R xpx = 0, xpy = 0, xpz = 0, xmx = 0, xmy = 0, xmz = 0, ypx = 0, ypy = 0, ypz = 0;
+        R ymx = 0, ymy = 0, ymz = 0, zpx = 0, zpy = 0, zpz = 0, zmx = 0, zmy = 0, zmz = 0;
+        auto load_u = [&]() {
+            const U* u0 = static_cast<const U*>(B.u_prev);
+            auto ld3 = [&](uint32_t c, R& a, R& b, R& d) {
+                const U* p = &at(u0, c * (uint32_t)US);
+                a = (R)p[0]; b = (R)p[1]; d = (R)p[2];
+            };
+            ld3(cell + 1, xpx, xpy, xpz); ld3(cell - 1, xmx, xmy, xmz);
+            ld3(cell + sy, ypx, ypy, ypz); ld3(cell - sy, ymx, ymy, ymz);
+            ld3(cell + sz, zpx, zpy, zpz); ld3(cell - sz, zmx, zmy, zmz);
+        };
+        // strain rate |S| = sqrt(2 S:S) by central differences
+        auto strain_rate = [&]() -> R {
+            const R h = R(0.5);
+            const R dux_dx = (xpx - xmx) * h, duy_dy = (ypy - ymy) * h, duz_dz = (zpz - zmz) * h;
+            const R dux_dy = (ypx - ymx) * h, dux_dz = (zpx - zmx) * h;
+            const R duy_dx = (xpy - xmy) * h, duy_dz = (zpy - zmy) * h;
+            const R duz_dx = (xpz - xmz) * h, duz_dy = (ypz - ymz) * h;
+            const R sxy = h * (dux_dy + duy_dx), sxz = h * (dux_dz + duz_dx), syz = h * (duy_dz + duz_dy);
+            const R s2 = dux_dx * dux_dx + duy_dy * duy_dy + duz_dz * duz_dz + R(2) * (sxy * sxy + sxz * sxz + syz * syz);
+            return __builtin_sqrt(R(2) * s2);
+        };
+        if (EARLY_U && strain) load_u();
 
         // ---------------- pull S(x, q) from the previous post-collision state
         R f[Q];
         if (P.first) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) f[q] = ldq<S, R>(fin_, n, q, cell, w_of<R>(q));
+            for (int q = 0; q < Q; ++q) f[q] = lin.ld(cell, (uint32_t)q * n, w_of<R>(q));
         } else if (!(m & M_SHELL)) {
             // interior: the source of q is cell - c_q; a uniform shifted base per direction
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const int off = cx_of(q) + cy_of(q) * nx + cz_of(q) * (int)sz;
-                f[q] = ldq<S, R>(fin_, n, q, cell - (uint32_t)off, w_of<R>(q));
+                f[q] = lin.ld(cell, (uint32_t)q * n - (uint32_t)off, w_of<R>(q));
             }
             if (m & M_LINKS) {   // half-way bounce-back: the cell's own opposite population
 #pragma unroll
                 for (int q = 1; q < Q; ++q)
-                    if ((m >> q) & 1u) f[q] = ldq<S, R>(fin_, n, opp_of(q), cell, w_of<R>(opp_of(q)));
+                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, w_of<R>(opp_of(q)));
             }
         } else if (m & M_PULL_INLET) {
 #pragma unroll
@@ -340,9 +397,9 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const R w = w_of<R>(q);
-                const R fo = ldq<S, R>(fin_, n, q, cell, w);
-                const R fi = ldq<S, R>(fin_, n, q, ci, w);
-                const R fi2 = ldq<S, R>(fin_, n, q, ci2, w);
+                const R fo = lin.ld(cell, (uint32_t)q * n, w);
+                const R fi = lin.ld(ci, (uint32_t)q * n, w);
+                const R fi2 = lin.ld(ci2, (uint32_t)q * n, w);
                 f[q] = fo + P.conv_c * (fi - fo) + P.conv_k2 * (fi - fi2);
             }
         } else {
@@ -364,26 +421,42 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
             }
         }
 
+        R smag = R(0);
+        if (EARLY_U && strain) smag = strain_rate();
+
         // ---------------- moments with sanitising (aeros/src/lbm.cpp:1061-1089)
         // Every population in [-1, 10] (which also excludes NaN/Inf) is the
         // common case; only otherwise are populations replaced and counted.
-        bool clean = true;
-#pragma unroll
-        for (int q = 0; q < Q; ++q) clean = clean && (f[q] >= R(-1)) && (f[q] <= R(10));
         R rho = R(0), ux = R(0), uy = R(0), uz = R(0);
-        int nanc = 0, badc = 0;
-        if (clean) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                // rho += f; u += f c with c in {-1,0,1}: the products by 0 only add
-                // signed zeros, so they are left out
-                const R fi = f[q];
-                rho += fi;
-                if (cx_of(q) > 0) ux += fi; else if (cx_of(q) < 0) ux -= fi;
-                if (cy_of(q) > 0) uy += fi; else if (cy_of(q) < 0) uy -= fi;
-                if (cz_of(q) > 0) uz += fi; else if (cz_of(q) < 0) uz -= fi;
+        for (int q = 0; q < Q; ++q) {
+            // rho += f; u += f c with c in {-1,0,1}: the products by 0 only add
+            // signed zeros, so they are left out
+            const R fi = f[q];
+            rho += fi;
+            if (cx_of(q) > 0) ux += fi; else if (cx_of(q) < 0) ux -= fi;
+            if (cy_of(q) > 0) uy += fi; else if (cy_of(q) < 0) uy -= fi;
+            if (cz_of(q) > 0) uz += fi; else if (cz_of(q) < 0) uz -= fi;
+        }
+        bool clean;
+        if constexpr (sizeof(R) == 4) {
+            // NaN-propagating minimum/maximum (three-operand on gfx950): a NaN
+            // fails both bounds, an infinity one of them
+            R lo = f[0], hi = f[0];
+#pragma unroll
+            for (int q = 1; q < Q; ++q) {
+                lo = __builtin_elementwise_minimum(lo, f[q]);
+                hi = __builtin_elementwise_maximum(hi, f[q]);
             }
+            clean = lo >= R(-1) && hi <= R(10);
         } else {
+            clean = true;
+#pragma unroll
+            for (int q = 0; q < Q; ++q) clean = clean && (f[q] >= R(-1)) && (f[q] <= R(10));
+        }
+        int nanc = 0, badc = 0;
+        if (!clean) {
+            rho = ux = uy = uz = R(0);
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 R fi = f[q];
@@ -406,31 +479,14 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
             const R ut = ux * ux + uy * uy + uz * uz;
             if (ut > R(0.09)) { const R s = __builtin_sqrt(R(0.09) / ut); ux *= s; uy *= s; uz *= s; }
         }
-        const bool inlet = m & M_INLET, outlet = m & M_OUTLET;
         if (inlet && !P.periodic_x) { rho = R(1); ux = P.in_u[0]; uy = P.in_u[1]; uz = P.in_u[2]; }
 
-        // ---------------- Smagorinsky from the previous step's velocity
-        // (aeros/src/lbm.cpp:979-1036, 1126-1145)
-        R tau_eff = P.tau0, smag = R(0);
+        // ---------------- Smagorinsky eddy viscosity (aeros/src/lbm.cpp:1126-1145)
+        R tau_eff = P.tau0;
         if (LES && !inlet && !outlet) {
-            if (m & M_STRAIN) {
-                const U* u0 = static_cast<const U*>(B.u_prev);
-                auto ld3 = [&](uint32_t c, R& a, R& b, R& d) {
-                    const U* p = &at(u0, c * (uint32_t)US);
-                    a = (R)p[0]; b = (R)p[1]; d = (R)p[2];
-                };
-                R xpx, xpy, xpz, xmx, xmy, xmz, ypx, ypy, ypz, ymx, ymy, ymz, zpx, zpy, zpz, zmx, zmy, zmz;
-                ld3(cell + 1, xpx, xpy, xpz); ld3(cell - 1, xmx, xmy, xmz);
-                ld3(cell + sy, ypx, ypy, ypz); ld3(cell - sy, ymx, ymy, ymz);
-                ld3(cell + sz, zpx, zpy, zpz); ld3(cell - sz, zmx, zmy, zmz);
-                const R h = R(0.5);
-                const R dux_dx = (xpx - xmx) * h, duy_dy = (ypy - ymy) * h, duz_dz = (zpz - zmz) * h;
-                const R dux_dy = (ypx - ymx) * h, dux_dz = (zpx - zmx) * h;
-                const R duy_dx = (xpy - xmy) * h, duy_dz = (zpy - zmy) * h;
-                const R duz_dx = (xpz - xmz) * h, duz_dy = (ypz - ymz) * h;
-                const R sxy = h * (dux_dy + duy_dx), sxz = h * (dux_dz + duz_dx), syz = h * (duy_dz + duz_dy);
-                const R s2 = dux_dx * dux_dx + duy_dy * duy_dy + duz_dz * duz_dz + R(2) * (sxy * sxy + sxz * sxz + syz * syz);
-                smag = __builtin_sqrt(R(2) * s2);
+            if (!EARLY_U && strain) {
+                load_u();
+                smag = strain_rate();
             }
             if (fin(smag) && smag < R(10) && smag > R(1e-6)) {
                 R cs = P.cs;
@@ -473,12 +529,16 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
         }
 
         // ---------------- pairwise TRT/BGK relaxation (aeros/src/lbm.cpp:1204-1264)
-        // Inputs are finite on the clean path, so the relaxation's "non-finite
-        // input -> equilibrium" replacement only runs otherwise.
+        // A non-finite input population relaxes from its equilibrium; inputs
+        // are finite on the clean path, so the replacement runs only otherwise.
+        if (!clean) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (!fin(f[q])) f[q] = feq_rt<R>(q, rho, ex, ey, ez, es);
+        }
         {
             const R feq0 = feq_k<R, 0>(rho, ex, ey, ez, es);
-            R fi = f[0];
-            if (!clean && !fin(fi)) fi = feq0;
+            const R fi = f[0];
             R fn = fi - (fi - feq0) * inv_tau;
             if (has_force) fn = fn + (R(1) - R(0.5) * inv_tau) * guo<R, 0>(ex, ey, ez, Fx, Fy, Fz);
             f[0] = fn;
@@ -489,11 +549,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
             constexpr int qi = QI, qo = QI + 1;                                                     \
             const R feq = feq_k<R, qi>(rho, ex, ey, ez, es);                                        \
             const R feqo = feq_k<R, qo>(rho, ex, ey, ez, es);                                       \
-            R fi = f[qi], fo = f[qo];                                                               \
-            if (!clean) {                                                                           \
-                if (!fin(fi)) fi = feq;                                                             \
-                if (!fin(fo)) fo = feqo;                                                            \
-            }                                                                                       \
+            const R fi = f[qi], fo = f[qo];                                                         \
             const R fp = R(0.5) * (fi + fo), fm = R(0.5) * (fi - fo);                               \
             const R ep = R(0.5) * (feq + feqo), em = R(0.5) * (feq - feqo);                         \
             const R fpn = fp - inv_tau * (fp - ep);                                                 \
@@ -514,21 +570,29 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
         AEROS_PAIR(1) AEROS_PAIR(3) AEROS_PAIR(5) AEROS_PAIR(7) AEROS_PAIR(9)
         AEROS_PAIR(11) AEROS_PAIR(13) AEROS_PAIR(15) AEROS_PAIR(17)
 #undef AEROS_PAIR
-        // non-finite results fall back to equilibrium; count negatives
-        bool fin_all = true;
+        // non-finite results fall back to equilibrium (with bounded
+        // parameters a clean cell cannot produce one)
+        if (!clean || !P.bounded) {
+            bool fin_all = true;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) fin_all = fin_all && fin(f[q]);
-        if (!fin_all) {
+            for (int q = 0; q < Q; ++q) fin_all = fin_all && fin(f[q]);
+            if (!fin_all) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (!fin(f[q])) f[q] = feq_rt<R>(q, rho, ex, ey, ez, es);
+                for (int q = 0; q < Q; ++q)
+                    if (!fin(f[q])) f[q] = feq_rt<R>(q, rho, ex, ey, ez, es);
+            }
         }
-        int negc = 0;
+        {   // negative populations, counted per wave (ballot + popcount on the
+            // scalar unit) and credited to the wave's first active lane
+            const uint64_t live = __ballot(1);
+            int wn = 0;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) negc += f[q] < R(0);
+            for (int q = 0; q < Q; ++q) wn += __popcll(__ballot(f[q] < R(0)));
+            if (__lane_id() == (unsigned)(__ffsll((unsigned long long)live) - 1)) acc.neg += wn;
+        }
 
 #pragma unroll
-        for (int q = 0; q < Q; ++q) stq<S, R>(fout, n, q, cell, f[q], w_of<R>(q));
+        for (int q = 0; q < Q; ++q) lout.st(cell, (uint32_t)q * n, f[q], w_of<R>(q));
 
         // ---------------- velocity and diagnostics (aeros/src/lbm.cpp:1266-1305)
         R vx = ux, vy = uy, vz = uz;   // the stored velocity includes F/(2 rho)
@@ -557,7 +621,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
         acc.rho += (double)rho;
         acc.ke += (double)usqr;
         acc.tke += (double)tk;
-        acc.nan += nanc; acc.bad += badc; acc.neg += negc;
+        acc.nan += nanc; acc.bad += badc;
 
         // ---------------- D3Q7 temperature: pull, then BGK with the new velocity
         // (aeros/src/thermal_lbm.cpp:205-336).  x and z always wrap for this
