@@ -369,6 +369,16 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
             return __builtin_sqrt(R(2) * s2);
         };
         if (EARLY_U && strain) load_u();
+        // likewise the shifted temperature populations and the buoyancy of an
+        // interior cell (solid sources are replaced after the collision)
+        R gq[Q7], bq = R(0);
+        const bool early_g = THERM && EARLY_U && !(m & M_SHELL);
+        if (early_g) {
+            const R* g = static_cast<const R*>(B.g_in);
+#pragma unroll
+            for (int i = 0; i < Q7; ++i) gq[i] = at(g + (size_t)i * n - (cx_of(i) + cy_of(i) * nx + cz_of(i) * (int)sz), cell);
+            bq = at(static_cast<const R*>(B.b_in), cell);
+        }
 
         // ---------------- pull S(x, q) from the previous post-collision state
         R f[Q];
@@ -512,7 +522,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
         // ---------------- Guo forcing (aeros/src/lbm.cpp:1161-1176)
         R Fx = R(0), Fy = R(0), Fz = R(0);
         if (FORCE) { Fx = P.fx; Fy = P.fy; Fz = P.fz; }
-        if (THERM) Fy += at(static_cast<const R*>(B.b_in), cell);
+        if (THERM) Fy += early_g ? bq : at(static_cast<const R*>(B.b_in), cell);
         const bool has_force = (FORCE || THERM) && (Fx != R(0) || Fy != R(0) || Fz != R(0));
         R ex = ux, ey = uy, ez = uz, es = usqr;
         if (has_force) {
@@ -641,7 +651,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
                         gi[i] = (P.t_fixed && (wy <= 0 || wy >= ny - 1))
                                     ? (-go_ + R(2) * w7_of<R>(i) * (wy <= 0 ? P.t_hot : P.t_cold)) : go_;
                     } else {
-                        gi[i] = at(g + (size_t)i * n - off, cell);
+                        gi[i] = early_g ? gq[i] : at(g + (size_t)i * n - off, cell);
                     }
                 }
             } else {

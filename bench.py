@@ -12,15 +12,26 @@ Prints one JSON line.  `value` is MLUPS = cells * steps / seconds / 1e6 over
 the timed steps; `roofline` uses the step kernel's average duration from HIP
 events on the solver's stream and the algorithmic 177 bytes per cell update
 (19 FP32 loads + 19 stores + previous-step velocity read and current write for
-the Smagorinsky stencil + 1 flag byte); `cpu_baseline` times the CPU
+the Smagorinsky stencil + 1 flag byte).  `roofline.traffic` is the HBM bytes
+per step-kernel launch from two rocprofv3 counter passes (FETCH_SIZE, then
+WRITE_SIZE) over a short run of the same case, each scaled by what the same
+counter reads for a plain copy kernel of known size in the same process.
+`secondary` holds the other precisions and cases (33.5M cells FP32/FP16/BF16,
+the FP64 sphere, the coupled thermal lattice); `cpu_baseline` times the CPU
 restatement on the host cores for a bounded number of steps of the same case.
 """
 from __future__ import annotations
 
 import argparse
+import csv
+import ctypes as C
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 from pathlib import Path
 
@@ -29,16 +40,19 @@ sys.path.insert(0, str(ROOT / "aeros-engine_amd"))
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL = {("fp32", True): 177, ("fp32", False): 153, ("fp16", True): 89, ("bf16", True): 89,
-                  ("fp64", False): 305, ("fp64", True): 353}
+                  ("fp64", False): 305, ("fp64", True): 353, ("thermal", False): 217}
+COPY_BYTES = 256 << 20   # the counter-scaling copy
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(A, W, cfg, steps, warmup, label):
+def run_gpu(A, cfg, steps, warmup, label, T=None):
     t0 = time.time()
     L = A.Lattice(A.default_config(**cfg))
+    if T is not None:
+        L.thermal_upload(T)
     log(f"[{label}] lattice {L.dims} ready in {time.time() - t0:.1f} s")
     L.step(warmup)
     L.kernel_timing(-1)
@@ -51,12 +65,20 @@ def run_gpu(A, W, cfg, steps, warmup, label):
     cells = L.ncells
     res = dict(dims=L.dims, cells=cells, seconds=dt, mlups=cells * steps / dt / 1e6,
                kernel_ms=kms / max(1, launches), launches=launches, avg_rho=sc["avg_rho"],
-               max_u=sc["max_u_lb"], step=sc["step"])
+               max_u=sc["max_u_lb"], step=sc["step"], nusselt=sc.get("nusselt"))
     L.close()
+    log(f"[{label}] {res['mlups']:.1f} MLUPS, step kernel {res['kernel_ms']:.5f} ms")
     return res
 
 
-def cpu_baseline(W, cfg, budget_s=12.0, max_steps=40):
+def line(r, bpc):
+    return {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
+            "pct_hbm": round(r["mlups"] * 1e6 * bpc / (HBM_PEAK_GBS * 1e9), 4),
+            "achieved_gbs": round(bpc * r["cells"] / (r["kernel_ms"] * 1e-3) / 1e9, 1),
+            "bytes_per_cell": bpc, "cells": r["cells"]}
+
+
+def cpu_baseline(cfg, budget_s=12.0, max_steps=40):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     O.build()
@@ -77,14 +99,86 @@ def cpu_baseline(W, cfg, budget_s=12.0, max_steps=40):
                        f"oracle (C++ restatement, OpenMP {threads} threads)")
 
 
+# ------------------------------------------------------------------ HBM traffic
+def device_copy(nbytes, reps):
+    """A plain float4 copy kernel over nbytes, reps times (aeros-engine_amd/csrc/copy_probe.hip)."""
+    lib = C.CDLL(str(ROOT / "aeros-engine_amd" / "build" / "libaeros_copyprobe.so"))
+    lib.aeros_probe_copy.restype = C.c_int
+    lib.aeros_probe_copy.argtypes = [C.c_size_t, C.c_int]
+    rc = lib.aeros_probe_copy(nbytes, reps)
+    if rc:
+        raise RuntimeError(f"copy probe failed ({rc})")
+
+
+def pmc_child(steps):
+    """What each counter pass runs: the headline case for a few steps, then the copy."""
+    import aeros_lbm as A
+    import workloads as W
+    L = A.Lattice(A.default_config(**W.c2_config(A.voxelize)))
+    L.step(steps)
+    L.close()
+    device_copy(COPY_BYTES, 4)
+    print("pmc child done", flush=True)
+
+
+def read_counter(d, name):
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    step, copy = [], []
+    for f in files:
+        for r in csv.DictReader(open(f)):
+            if r["Counter_Name"] != name:
+                continue
+            v = float(r["Counter_Value"])
+            if "k_step" in r["Kernel_Name"]:
+                step.append((int(r["Dispatch_Id"]), v))
+            elif "k_copy_probe" in r["Kernel_Name"]:
+                copy.append(v)
+    step.sort()
+    step = [v for _, v in step[1:]]   # the first launch collides in place
+    if not step or not copy:
+        raise RuntimeError(f"{name}: no step or copy dispatches in {d}")
+    return sum(step) / len(step), sum(copy) / len(copy)
+
+
+def measure_traffic(steps=20, timeout=240):
+    """HBM bytes per step-kernel launch from FETCH_SIZE and WRITE_SIZE passes."""
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None, "rocprofv3 not found"
+    res = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="aeros_pmc_", dir="/tmp")
+        cmd = [exe, "--pmc", ctr, "-d", d, "-o", "run", "-f", "csv", "--", sys.executable, str(ROOT / "bench.py"),
+               "--pmc-child", str(steps)]
+        log(f"[traffic] counter pass {ctr}")
+        try:
+            r = subprocess.run(cmd, cwd="/tmp", env={**os.environ, "TMPDIR": "/tmp"}, timeout=timeout,
+                               capture_output=True, text=True)
+            if "pmc child done" not in r.stdout:
+                return None, f"{ctr} pass failed (rc {r.returncode}): {r.stderr[-300:]}"
+            kib, copy_kib = read_counter(d, ctr)
+        except (subprocess.TimeoutExpired, RuntimeError, OSError) as e:
+            return None, f"{ctr} pass: {e}"
+        finally:
+            shutil.rmtree(d, ignore_errors=True)
+        scale = (COPY_BYTES / 1024.0) / copy_kib
+        res[ctr] = dict(kib=kib, copy_kib=copy_kib, scale=scale, bytes=kib * 1024.0 * scale)
+    return res, None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-secondary", action="store_true", help="skip the 33.5M-cell precision lines")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the other precisions and cases")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.pmc_child:
+        pmc_child(args.pmc_child)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
               "see DESIGN.md for how the lattice would shard by x-slabs.", file=sys.stderr)
@@ -101,7 +195,7 @@ def main():
     t0 = time.time()
     c2 = W.c2_config(A.voxelize)
     log(f"C2 mask voxelized in {time.time() - t0:.1f} s ({int(c2['voxels'].sum())} solid cells)")
-    r = run_gpu(A, W, c2, args.steps, args.warmup, "c2 fp32")
+    r = run_gpu(A, c2, args.steps, args.warmup, "c2 fp32")
     bpc = BYTES_PER_CELL[("fp32", True)]
     achieved = bpc * r["cells"] / (r["kernel_ms"] * 1e-3) / 1e9
     out = {
@@ -130,17 +224,35 @@ def main():
         sec = {}
         for prec in ("fp32", "fp16", "bf16"):
             c4 = W.c2_config(A.voxelize, scale=2.0, precision=prec)
-            rr = run_gpu(A, W, c4, 200, 20, f"c4 {prec}")
-            b = BYTES_PER_CELL[(prec, True)]
-            sec[prec] = {"mlups": round(rr["mlups"], 1), "kernel_ms": round(rr["kernel_ms"], 4),
-                         "pct_hbm": round(rr["mlups"] * 1e6 * b / (HBM_PEAK_GBS * 1e9), 4),
-                         "achieved_gbs": round(b * rr["cells"] / (rr["kernel_ms"] * 1e-3) / 1e9, 1),
-                         "bytes_per_cell": b}
-        sec["fp16_over_fp32"] = round(sec["fp16"]["mlups"] / sec["fp32"]["mlups"], 3)
-        sec["bf16_over_fp32"] = round(sec["bf16"]["mlups"] / sec["fp32"]["mlups"], 3)
-        out["secondary"] = {"workload": "512x256x256 (33.5M cells), ellipsoid scaled x2, Re 1000", **sec}
+            sec[f"c4_{prec}"] = line(run_gpu(A, c4, 200, 20, f"c4 {prec}"), BYTES_PER_CELL[(prec, True)])
+        sec["fp16_over_fp32"] = round(sec["c4_fp16"]["mlups"] / sec["c4_fp32"]["mlups"], 3)
+        sec["bf16_over_fp32"] = round(sec["c4_bf16"]["mlups"] / sec["c4_fp32"]["mlups"], 3)
+        sec["c1_fp64"] = line(run_gpu(A, W.c1_sphere_fp64(), 1000, 50, "c1 fp64"), BYTES_PER_CELL[("fp64", False)])
+        c3cfg, c3T = W.c3_rayleigh_benard()
+        sec["c3_thermal_fp32"] = line(run_gpu(A, c3cfg, 500, 20, "c3 thermal", T=c3T), BYTES_PER_CELL[("thermal", False)])
+        sec["workloads"] = {"c4": "512x256x256 (33.5M cells), ellipsoid scaled x2, Re 1000, TRT+LES",
+                            "c1": "96x48x48 FP64 BGK sphere D=12, U0 0.05, zero-gradient outflow",
+                            "c3": "128x64x128 coupled D3Q19+D3Q7 FP32, Rayleigh-Benard Ra 2500 Pr 0.71"}
+        out["secondary"] = sec
+    if not args.no_traffic:
+        tr, err = measure_traffic()
+        if tr:
+            total = tr["FETCH_SIZE"]["bytes"] + tr["WRITE_SIZE"]["bytes"]
+            out["roofline"]["traffic"] = round(total)
+            out["roofline"]["traffic_detail"] = {
+                "read_bytes": round(tr["FETCH_SIZE"]["bytes"]), "write_bytes": round(tr["WRITE_SIZE"]["bytes"]),
+                "bytes_per_cell": round(total / r["cells"], 1),
+                "over_algorithmic": round(total / (bpc * r["cells"]), 3),
+                "fetch_kib_per_launch": round(tr["FETCH_SIZE"]["kib"], 1),
+                "write_kib_per_launch": round(tr["WRITE_SIZE"]["kib"], 1),
+                "copy_fetch_kib": round(tr["FETCH_SIZE"]["copy_kib"], 1), "copy_write_kib": round(tr["WRITE_SIZE"]["copy_kib"], 1),
+                "copy_bytes": COPY_BYTES,
+                "fetch_scale": round(tr["FETCH_SIZE"]["scale"], 4), "write_scale": round(tr["WRITE_SIZE"]["scale"], 4)}
+        else:
+            log(f"[traffic] not measured: {err}")
+            out["roofline"]["traffic_note"] = err
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(W, c2)
+        out["cpu_baseline"] = cpu_baseline(c2)
     print(json.dumps(out), flush=True)
 
 
