@@ -181,7 +181,7 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
-              "see DESIGN.md for how the lattice would shard by x-slabs.", file=sys.stderr)
+              "the lattice would shard by x-slabs with a five-population halo per face.", file=sys.stderr)
         sys.exit(2)
 
     import aeros_lbm as A

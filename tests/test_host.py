@@ -6,6 +6,7 @@ loudly with AEROS_ERR_NO_DEVICE.
 """
 import ctypes as C
 import re
+import shutil
 import subprocess
 from pathlib import Path
 
@@ -152,3 +153,18 @@ def test_voxelizer_brute_force_tiny(oracle):
     Z, Y, X = np.meshgrid(c, c, c, indexing="ij")
     inside = (np.abs(X) < x) & (np.abs(Y) < y) & (np.abs(Z) < z)
     np.testing.assert_array_equal(vox.astype(bool), inside)
+
+
+REFERENCE_SRC = Path("/root/reference/aeros/src")
+
+
+@pytest.mark.skipif(not REFERENCE_SRC.is_dir() or shutil.which("g++") is None,
+                    reason="the reference's headers are only present where the reference is mounted")
+def test_shim_compiles_against_reference_headers():
+    """The reference-side binding compiles inside the reference tree (its lbm.h, globals.h, ...)."""
+    ref = REFERENCE_SRC.parent.parent
+    cmd = ["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", f"-I{REFERENCE_SRC}", f"-I{ref / 'libs' / 'glm'}",
+           f"-I{ref / 'libs' / 'glad' / 'include'}", f"-I{ref / 'libs' / 'glfw' / 'include'}",
+           f"-I{ROOT / 'include'}", str(ROOT / "aeros-engine_amd" / "shim" / "lbm_hip_shim.cpp")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
