@@ -186,6 +186,36 @@ def set_voxels(c: Config, vox) -> None:
 _lib = None
 
 
+class ServeConfig(C.Structure):
+    """aeros_serve_config (include/aeros_serve.h)."""
+    _fields_ = [("struct_size", C.c_int32), ("precision", C.c_int32), ("memory_budget", C.c_int64),
+                ("max_cases", C.c_int32), ("round_steps", C.c_int32), ("cd_tolerance", C.c_double),
+                ("max_steps", C.c_int32), ("cv_half", C.c_float), ("device", C.c_int32)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("state", C.c_int32), ("steps", C.c_int32), ("rounds", C.c_int32), ("resets", C.c_int32),
+                ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("error", C.c_int32),
+                ("cd", C.c_double), ("cd_prev", C.c_double), ("area", C.c_float), ("nusselt", C.c_float),
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double), ("bytes", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("rounds", C.c_int64),
+                ("launches", C.c_int64), ("resident", C.c_int32), ("queued", C.c_int32),
+                ("peak_resident", C.c_int32), ("bytes_in_use", C.c_int64), ("peak_bytes", C.c_int64),
+                ("busy_seconds", C.c_double), ("cell_steps", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+REQ_STATES = {0: "queued", 1: "running", 2: "done", 3: "limit", 4: "failed"}
+
+
 def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
     """Load libaeros_lbm.so and declare its prototypes.  Raises if it is missing."""
     global _lib
@@ -227,6 +257,14 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
         "aeros_release_all": (None, []),
         "aeros_lbm_kernel_timing": (C.c_int, [vp, C.c_int32, P(C.c_double), P(C.c_int64)]),
         "setVoxelData": (None, [P(C.c_int), P(C.c_float), C.c_int, C.c_int, C.c_int] + [C.c_float] * 6),
+        "aeros_serve_config_defaults": (None, [P(ServeConfig)]),
+        "aeros_serve_create": (vp, [P(ServeConfig), P(C.c_int)]),
+        "aeros_serve_submit": (C.c_int, [vp, P(Config), P(C.c_int64)]),
+        "aeros_serve_poll": (C.c_int, [vp, C.c_int64, P(ServeResult)]),
+        "aeros_serve_wait_all": (C.c_int, [vp, C.c_double]),
+        "aeros_serve_clock": (C.c_double, [vp]),
+        "aeros_serve_stats_get": (C.c_int, [vp, P(ServeStats)]),
+        "aeros_serve_destroy": (C.c_int, [vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -409,3 +447,59 @@ def voxelize(tris, dims, box_min, cell):
     _check(lib.aeros_voxelize(tris.ctypes.data_as(C.POINTER(C.c_float)), tris.shape[0], nx, ny, nz, mn, cs,
                               out.ctypes.data_as(C.POINTER(C.c_int32))), "aeros_voxelize")
     return out.reshape(nz, ny, nx)
+
+
+class Server:
+    """Served mode (include/aeros_serve.h): a server thread advances many
+    resident cases in batched launches; submit() queues a case, poll() and
+    wait_all() report results."""
+
+    def __init__(self, **overrides):
+        self.lib = load_library()
+        cfg = ServeConfig()
+        self.lib.aeros_serve_config_defaults(C.byref(cfg))
+        for k, v in overrides.items():
+            if k == "precision" and isinstance(v, str):
+                v = PRECISIONS[v]
+            setattr(cfg, k, v)
+        self.config = cfg
+        err = C.c_int(0)
+        h = self.lib.aeros_serve_create(C.byref(cfg), C.byref(err))
+        if not h:
+            raise AerosError(f"aeros_serve_create failed: {ERRORS.get(err.value, err.value)} {last_error()}")
+        self.h = C.c_void_p(h)
+        self._keep = {}
+
+    def submit(self, config: Config) -> int:
+        rid = C.c_int64(0)
+        _check(self.lib.aeros_serve_submit(self.h, C.byref(config), C.byref(rid)), "aeros_serve_submit")
+        return rid.value
+
+    def poll(self, rid: int) -> dict:
+        r = ServeResult()
+        _check(self.lib.aeros_serve_poll(self.h, int(rid), C.byref(r)), "aeros_serve_poll")
+        d = r.as_dict()
+        d["state_name"] = REQ_STATES.get(d["state"], str(d["state"]))
+        return d
+
+    def wait_all(self, timeout_s: float = 0.0) -> bool:
+        return self.lib.aeros_serve_wait_all(self.h, float(timeout_s)) == 0
+
+    def clock(self) -> float:
+        return self.lib.aeros_serve_clock(self.h)
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        _check(self.lib.aeros_serve_stats_get(self.h, C.byref(st)), "aeros_serve_stats_get")
+        return st.as_dict()
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.aeros_serve_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -403,7 +403,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         const auto t0 = std::chrono::steady_clock::now();
         StepParams<R> P = params();
         const int start = cur;
-        bool was_pre = pre;
+        const bool was_pre = pre;
         if (timing) {
             const int rc = ensure_events((size_t)nsteps * 2);
             if (rc) return rc;
@@ -418,9 +418,8 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             HIP_TRY((launch_step<R, S, U>(P, B, grid, st)));
             if (timing) HIP_TRY(hipEventRecord(events[(size_t)i * 2 + 1], st));
         }
-        Status S_{};
-        HIP_TRY(hipMemcpyAsync(&S_, status.p, sizeof S_, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
+        int rc = finish(start, nsteps, was_pre);
+        if (rc) return rc;
         if (timing) {
             for (int i = 0; i < nsteps; ++i) {
                 float ms = 0.0f;
@@ -429,10 +428,26 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             }
             timed_launches += nsteps;
         }
+        rc = post_pass();
+        if (rc) return rc;
+        rc = thermal_diag();
+        if (rc) return rc;
+        time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        return AEROS_OK;
+    }
+
+    // After nsteps launches from step index `start`: read the status and, if
+    // the loop broke, finish the broken step and reset the fluid as the
+    // reference does.  Shared by single-lattice stepping and served rounds.
+    int finish(int start, int nsteps, bool was_pre) {
+        Status S_{};
+        HIP_TRY(hipMemcpyAsync(&S_, status.p, sizeof S_, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
         if (S_.abort == 0) {
             cur = start + nsteps;
             pre = false;
         } else {
+            StepParams<R> P = params();
             // the loop broke at reference step s (aeros/src/lbm.cpp:1313-1319, 1574-1580)
             const int sidx = S_.abort_step;
             const int i = sidx - start - 1;              // index of that kernel within this call
@@ -461,12 +476,29 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             HIP_TRY(hipMemcpyAsync(status.p, &S_, sizeof S_, hipMemcpyHostToDevice, st));
         }
         host_status = S_;
-        int rc = post_pass();
-        if (rc) return rc;
-        rc = thermal_diag();
-        if (rc) return rc;
-        time_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
         return AEROS_OK;
+    }
+
+    // Served mode: this lattice's entry of a batched launch, on nblocks
+    // blocks (at most `grid`, the size of the partials buffer).
+    void batch_entry(BatchCase<R>& bc, uint32_t nblocks) {
+        bc.P = params();
+        bc.P.first = bc.P.write_fields = bc.P.dry = 0;
+        bc.buf[0] = buffers(0, false, false);
+        bc.buf[1] = buffers(1, false, false);
+        bc.start = cur;
+        bc.pre = pre ? 1 : 0;
+        bc.nblocks = std::min<uint32_t>(nblocks, (uint32_t)grid);
+        bc.pad = 0;
+    }
+    int64_t device_bytes() const {
+        int64_t b = 0;
+        for (const DevBuf* d : {&f[0], &f[1], &mask, &solid_u8, &u[0], &u[1], &g[0], &g[1], &T[0], &T[1], &buoy[0],
+                                &buoy[1], &parts, &status, &prev_rho, &scratch})
+            b += (int64_t)d->bytes;
+        for (const DevBuf& d : fld) b += (int64_t)d.bytes;
+        for (const DevBuf& d : world) b += (int64_t)d.bytes;
+        return b;
     }
 
     // world velocity, pressure and the sampled residuals (aeros/src/lbm.cpp:1602-1705)
@@ -485,8 +517,13 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         while (n / stride > 262144) stride *= 2;
         const bool valid = prev_valid && prev_step >= 0 && cur >= prev_step;
         float* out = reinterpret_cast<float*>(scratch.as<char>() + 64);
-        hipLaunchKernelGGL((k_post_scalars<R>), dim3(1), dim3(BLOCK), 0, st, world[3].as<R>(), fld[0].as<R>(),
-                           prev_rho.as<float>(), n, stride, (int)valid, out);
+        PostPart* pparts = reinterpret_cast<PostPart*>(scratch.as<char>() + 4096);
+        const int64_t ns = (n + stride - 1) / stride;
+        const int pgrid = (int)std::max<int64_t>(1, std::min<int64_t>((ns + BLOCK - 1) / BLOCK, kPostBlocks));
+        hipLaunchKernelGGL((k_post_scalars<R>), dim3(pgrid), dim3(BLOCK), 0, st, world[3].as<R>(), fld[0].as<R>(),
+                           prev_rho.as<float>(), n, stride, (int)valid, pparts);
+        HIP_TRY(hipGetLastError());
+        hipLaunchKernelGGL(k_post_fold, dim3(1), dim3(BLOCK), 0, st, pparts, pgrid, (int)valid, out);
         HIP_TRY(hipGetLastError());
         struct { unsigned long long bad; char pad[56]; float v[4]; } hb;
         HIP_TRY(hipMemcpyAsync(&hb, scratch.p, sizeof hb, hipMemcpyDeviceToHost, st));
@@ -989,3 +1026,5 @@ void setVoxelData(const int* voxel, const float* dist, int nx, int ny, int nz, f
 }
 
 }  // extern "C"
+
+#include "lbm_serve.inc"

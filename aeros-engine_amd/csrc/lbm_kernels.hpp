@@ -47,13 +47,23 @@ template <class R> __device__ __host__ inline R w7_of(int q) { return q == 0 ? R
 // ------------------------------------------------------------ storage
 struct bf16_t { uint16_t bits; };
 
+// Every solver array lives in device global memory.  Accesses go through
+// global-address-space references, so pointers that reach a kernel through
+// memory (served mode's per-case tables, which are generic pointers) still
+// use global, not flat, loads and stores.
+#define AEROS_GLOBAL __attribute__((address_space(1)))
+template <class T> using gref = AEROS_GLOBAL T&;
+template <class T> using gcref = const AEROS_GLOBAL T&;
+
 // Element i of an array through a 32-bit byte offset: with a uniform base the
 // load or store uses the scalar-base + 32-bit vector-offset address form.
-template <class T> __device__ inline const T& at(const T* base, uint32_t i) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + i * (uint32_t)sizeof(T));
+template <class T> __device__ inline gcref<T> at(const T* base, uint32_t i) {
+    const AEROS_GLOBAL char* b = (const AEROS_GLOBAL char*)base;
+    return *(const AEROS_GLOBAL T*)(b + i * (uint32_t)sizeof(T));
 }
-template <class T> __device__ inline T& at(T* base, uint32_t i) {
-    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + i * (uint32_t)sizeof(T));
+template <class T> __device__ inline gref<T> at(T* base, uint32_t i) {
+    AEROS_GLOBAL char* b = (AEROS_GLOBAL char*)base;
+    return *(AEROS_GLOBAL T*)(b + i * (uint32_t)sizeof(T));
 }
 
 template <class S, class R> struct Pop;
@@ -226,8 +236,8 @@ __device__ inline void block_reduce(RedLds& L, int tid, Partial& p) {
 // The last block to finish folds the per-block partials in block order and
 // applies the reference's end-of-step logic.
 template <class R>
-__device__ void finalize_step(const StepParams<R>& P, Partial* parts, Status* st, RedLds& L, int tid) {
-    const int nb = gridDim.x;
+__device__ __forceinline__ void finalize_step(const StepParams<R>& P, Partial* parts, Status* st, RedLds& L, int tid,
+                                              int nb) {
     double a = 0, b = 0, c = 0;
     float m = 0.0f;
     int64_t n0 = 0, n1 = 0, n2 = 0, n3 = 0, n4 = 0, n5 = 0;
@@ -307,12 +317,20 @@ struct StepBuffers {
 template <class U> struct VelAos { static constexpr int stride = 3; };
 template <> struct VelAos<_Float16> { static constexpr int stride = 4; };
 
+// Flags of one launch: the state holds pre-collision populations (collide
+// only), write the output fields, re-run of an aborted step (no abort check,
+// no reduction).
+struct StepFlags {
+    int first, write_fields, dry;
+};
+
+// One lattice step over blocks blk of nblk (the whole grid for a single
+// lattice, the lattice's share of the launch in served mode).
 template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
-__global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) {
-    __shared__ RedLds L;
-    __shared__ int is_last;
+__device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuffers& B, const StepFlags F,
+                                          const uint32_t blk, const uint32_t nblk, RedLds& L, int& is_last) {
     Status* st = B.status;
-    if (!P.dry && st->abort) return;   // an earlier step of this call broke the loop
+    if (!F.dry && st->abort) return;   // an earlier step of this call broke the loop
 
     const S* fin_ = static_cast<const S*>(B.f_in);
     const uint32_t n = (uint32_t)P.n;
@@ -328,8 +346,8 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
     // Each iteration issues the next cell's mask load first, then the
     // Smagorinsky stencil and the populations back to back, so a cell costs
     // one memory round trip.
-    const uint32_t stride = gridDim.x * BLOCK;
-    uint32_t cell = blockIdx.x * BLOCK + tid;
+    const uint32_t stride = nblk * BLOCK;
+    uint32_t cell = blk * BLOCK + tid;
     uint32_t m_next = cell < n ? at(B.mask, cell) : 0u;
     for (; cell < n; cell += stride) {
         const uint32_t m = m_next;
@@ -350,7 +368,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
         auto load_u = [&]() {
             const U* u0 = static_cast<const U*>(B.u_prev);
             auto ld3 = [&](uint32_t c, R& a, R& b, R& d) {
-                const U* p = &at(u0, c * (uint32_t)US);
+                const AEROS_GLOBAL U* p = &at(u0, c * (uint32_t)US);
                 a = (R)p[0]; b = (R)p[1]; d = (R)p[2];
             };
             ld3(cell + 1, xpx, xpy, xpz); ld3(cell - 1, xmx, xmy, xmz);
@@ -382,7 +400,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
 
         // ---------------- pull S(x, q) from the previous post-collision state
         R f[Q];
-        if (P.first) {
+        if (F.first) {
 #pragma unroll
             for (int q = 0; q < Q; ++q) f[q] = lin.ld(cell, (uint32_t)q * n, w_of<R>(q));
         } else if (!(m & M_SHELL)) {
@@ -611,7 +629,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
             vx = ux + Fx * hi; vy = uy + Fy * hi; vz = uz + Fz * hi;
         }
         if (LES) {
-            U* un = &at(static_cast<U*>(B.u_next), cell * (uint32_t)US);
+            AEROS_GLOBAL U* un = &at(static_cast<U*>(B.u_next), cell * (uint32_t)US);
             un[0] = (U)vx; un[1] = (U)vy; un[2] = (U)vz;
             if (US == 4) un[3] = (U)0;
         }
@@ -619,7 +637,7 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
         R tk = tl * P.vel_scale2;
         if (!fin(tk) || tk < R(0)) tk = R(0);
         if (tk > R(1.0e6)) tk = R(1.0e6);
-        if (P.write_fields) {
+        if (F.write_fields) {
             at(static_cast<R*>(B.fld[0]), cell) = rho;
             at(static_cast<R*>(B.fld[1]), cell) = vx;
             at(static_cast<R*>(B.fld[2]), cell) = vy;
@@ -699,22 +717,61 @@ __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) 
     }
 
     // ---------------- fixed-order reduction and end-of-step logic
-    if (P.dry) return;
+    if (F.dry) return;
     // max sqrt(usqr) == sqrt(max usqr): correctly rounded sqrt and the cast
     // to float are monotone, so the block maximum is unchanged.
     acc.maxv = (float)__builtin_sqrt(maxu2);
     block_reduce(L, tid, acc);
     if (tid == 0) {
-        B.parts[blockIdx.x] = acc;
+        B.parts[blk] = acc;
         __threadfence();
         const uint32_t t = atomicAdd(&st->ticket, 1u);
-        is_last = (t == gridDim.x - 1);
+        is_last = (t == nblk - 1);
     }
     __syncthreads();
     if (is_last) {
         __threadfence();
-        finalize_step<R>(P, B.parts, st, L, tid);
+        finalize_step<R>(P, B.parts, st, L, tid, (int)nblk);
     }
+}
+
+template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
+__global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) {
+    __shared__ RedLds L;
+    __shared__ int is_last;
+    step_grid<R, S, U, LES, THERM, FORCE>(P, B, StepFlags{P.first, P.write_fields, P.dry}, blockIdx.x, gridDim.x, L,
+                                          is_last);
+}
+
+// ------------------------------------------------------------ served mode
+// Many lattices advance in one launch: block b works on lattice map[b].ci as
+// its block map[b].blk of cases[ci].nblocks.  A lattice's block count depends
+// on its size only, so its arithmetic, reductions included, is the same alone
+// and in any mix.
+template <class R> struct BatchCase {
+    StepParams<R> P;          // first / write_fields / dry are set per launch
+    StepBuffers buf[2];       // by parity of the step's output (f_in of a first step is f_out)
+    int32_t start;            // step index before the round
+    int32_t pre;              // buffers hold pre-collision populations
+    uint32_t nblocks;
+    uint32_t pad;
+};
+struct BlockMap {
+    uint32_t ci, blk;
+};
+
+template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
+__global__ __launch_bounds__(BLOCK) void k_step_batch(const BatchCase<R>* cases, const BlockMap* map, int i,
+                                                      int nsteps) {
+    __shared__ RedLds L;
+    __shared__ int is_last;
+    const BlockMap bm = map[blockIdx.x];
+    const BatchCase<R>& C = cases[bm.ci];
+    const int k = C.start + i;
+    const StepFlags F{(C.pre && i == 0) ? 1 : 0, (i == nsteps - 1) ? 1 : 0, 0};
+    StepBuffers B = C.buf[k & 1];
+    if (F.first) B.f_in = B.f_out;
+    step_grid<R, S, U, LES, THERM, FORCE>(C.P, B, F, bm.blk, C.nblocks, L, is_last);
 }
 
 // ------------------------------------------------------------ other kernels
@@ -879,10 +936,18 @@ __global__ void k_post(const uint32_t* mask, const R* rho, const R* ux, const R*
 }
 
 // Sampled pressure range and rho residual (aeros/src/lbm.cpp:1662-1705):
-// cells 0, stride, 2 stride, ...; one block, fixed order.
+// cells 0, stride, 2 stride, ...  Each block reduces its grid-stride share
+// of the samples into one partial; k_post_fold folds the partials in block
+// order, so the result does not depend on scheduling.
+struct PostPart {
+    float mn, mx;
+    double ss;
+    int cnt, pad;
+};
+constexpr int kPostBlocks = 512;   // at most this many partials
 template <class R>
 __global__ void k_post_scalars(const R* p, const R* rho, float* prev, int64_t n, int64_t stride, int valid,
-                               float* out /* pmin, pmax, resid, nsampled */) {
+                               PostPart* parts) {
     __shared__ float smin[BLOCK], smax[BLOCK];
     __shared__ double ss[BLOCK];
     __shared__ int sn[BLOCK];
@@ -891,13 +956,38 @@ __global__ void k_post_scalars(const R* p, const R* rho, float* prev, int64_t n,
     double acc = 0.0;
     int cnt = 0;
     const int64_t ns = (n + stride - 1) / stride;
-    for (int64_t k = tid; k < ns; k += BLOCK) {
+    for (int64_t k = (int64_t)blockIdx.x * BLOCK + tid; k < ns; k += (int64_t)gridDim.x * BLOCK) {
         const int64_t c = k * stride;
         const float pv = (float)p[c];
         if (__builtin_isfinite(pv)) { mn = fminf(mn, pv); mx = fmaxf(mx, pv); }
         const float r = (float)rho[c];
         if (valid && __builtin_isfinite(r)) { const float d = r - prev[k]; acc += (double)d * (double)d; cnt++; }
         prev[k] = __builtin_isfinite(r) ? r : 1.0f;
+    }
+    smin[tid] = mn; smax[tid] = mx; ss[tid] = acc; sn[tid] = cnt;
+    for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        __syncthreads();
+        if (tid < off) {
+            smin[tid] = fminf(smin[tid], smin[tid + off]);
+            smax[tid] = fmaxf(smax[tid], smax[tid + off]);
+            ss[tid] += ss[tid + off];
+            sn[tid] += sn[tid + off];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) parts[blockIdx.x] = PostPart{smin[0], smax[0], ss[0], sn[0], 0};
+}
+__global__ void k_post_fold(const PostPart* parts, int nparts, int valid, float* out /* pmin, pmax, resid, nsampled */) {
+    __shared__ float smin[BLOCK], smax[BLOCK];
+    __shared__ double ss[BLOCK];
+    __shared__ int sn[BLOCK];
+    const int tid = threadIdx.x;
+    float mn = 1e30f, mx = -1e30f;
+    double acc = 0.0;
+    int cnt = 0;
+    for (int k = tid; k < nparts; k += BLOCK) {
+        mn = fminf(mn, parts[k].mn); mx = fmaxf(mx, parts[k].mx);
+        acc += parts[k].ss; cnt += parts[k].cnt;
     }
     smin[tid] = mn; smax[tid] = mx; ss[tid] = acc; sn[tid] = cnt;
     for (int off = BLOCK / 2; off > 0; off >>= 1) {
@@ -1077,6 +1167,24 @@ hipError_t launch_step(const StepParams<R>& P, const StepBuffers& B, int grid, h
                                 : launch_step_t<R, S, U, false, true, false>(P, B, grid, s);
     return force ? launch_step_t<R, S, U, false, false, true>(P, B, grid, s)
                  : launch_step_t<R, S, U, false, false, false>(P, B, grid, s);
+}
+
+// Served mode: one launch over the cases of one (LES, thermal) group; served
+// cases carry no body force.
+template <class R, class S, class U, bool LES, bool THERM>
+static hipError_t launch_batch_t(const BatchCase<R>* cases, const BlockMap* map, int nblocks, int i, int nsteps,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL((k_step_batch<R, S, U, LES, THERM, false>), dim3(nblocks), dim3(BLOCK), 0, s, cases, map, i,
+                       nsteps);
+    return hipGetLastError();
+}
+template <class R, class S, class U>
+hipError_t launch_batch(bool les, bool thermal, const BatchCase<R>* cases, const BlockMap* map, int nblocks, int i,
+                        int nsteps, hipStream_t s) {
+    if (les) return thermal ? launch_batch_t<R, S, U, true, true>(cases, map, nblocks, i, nsteps, s)
+                            : launch_batch_t<R, S, U, true, false>(cases, map, nblocks, i, nsteps, s);
+    return thermal ? launch_batch_t<R, S, U, false, true>(cases, map, nblocks, i, nsteps, s)
+                   : launch_batch_t<R, S, U, false, false>(cases, map, nblocks, i, nsteps, s);
 }
 
 }  // namespace aeros

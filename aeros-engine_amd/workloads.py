@@ -221,3 +221,57 @@ def c3_rayleigh_benard(nx=128, ny=64, nz=128, seed=2024, rayleigh=2500.0, prandt
     T[:, 0, :] = 1.0
     T[:, ny - 1, :] = 0.0
     return cfg, T.astype(f32)
+
+
+def ellipsoid_in_box(nx, ny, nz, centre, semi) -> np.ndarray:
+    """Ellipsoid by the cell-centre test sum((x - c)^2 / a^2) <= 1 in lattice units."""
+    z, y, x = np.meshgrid(np.arange(nz, dtype=f32) + f32(0.5), np.arange(ny, dtype=f32) + f32(0.5),
+                          np.arange(nx, dtype=f32) + f32(0.5), indexing="ij")
+    q = sum(((v - f32(c)) / f32(a)) ** 2 for v, c, a in zip((x, y, z), centre, semi))
+    return (q <= f32(1.0)).astype(np.int32)
+
+
+def c5_request(rng, u0=0.05, thermal_frac=0.3, size=None, tau_min=0.6, dcap=0.3, xc=1.0 / 3.0) -> tuple[dict, dict]:
+    """One served-mode request: grid on the line 32^3 .. 160x80x80, a sphere or an
+    ellipsoid (3:2:2) of diameter U[6, 20] cells (at most dcap of the cross-section) at a
+    third of the length, Re U[50, 500], and with probability thermal_frac a thermal
+    lattice with Ra U[1800, 5000], Pr 0.71, hot bottom / cold top walls.  TRT +
+    Smagorinsky, zero-gradient outflow, U0 = 0.05.
+
+    tau = 1/2 + 3 U0 D / Re, raised to tau_min where it falls below: with the
+    reference's open boundaries these small lattices diverge within a few hundred
+    steps from an impulsive start below tau ~ 0.6 (measured), so the realised Reynolds
+    number is 3 U0 D / (tau - 1/2).  The drag is measured on a control volume of 0.40 of
+    the domain around its centre, which encloses the body at every size."""
+    t = rng.uniform(0.0, 1.0) if size is None else size
+    nx = int(round(32 + 128 * t))
+    ny = nz = int(round(32 + 48 * t))
+    d = min(float(rng.uniform(6.0, 20.0)), dcap * ny)
+    ellip = bool(rng.uniform() < 0.5)
+    re = float(rng.uniform(50.0, 500.0))
+    thermal = bool(rng.uniform() < thermal_frac)
+    ra = float(rng.uniform(1800.0, 5000.0)) if thermal else 0.0
+    centre = (nx * xc, ny / 2.0, nz / 2.0)
+    if ellip:
+        vox = ellipsoid_in_box(nx, ny, nz, centre, (0.75 * d, 0.5 * d, 0.5 * d))
+    else:
+        vox = sphere_in_box(nx, ny, nz, centre, d)
+    tau = max(tau_min, 0.5 + 3.0 * u0 * d / re)
+    cfg = dict(voxels=vox, **lattice_box(nx, ny, nz), voxel_resolution=nx, max_dim=d, flow_speed=20.0, U0=u0,
+               tau=float(tau), use_les=1, use_trt=1, smagorinsky_c=0.1, auto_viscosity=0, inlet_turbulence=0.0,
+               convective_outlet=0, body_force=(0.0, 0.0, 0.0))
+    if thermal:
+        cfg.update(thermal=1, rayleigh=ra, prandtl=0.71, fixed_wall_temps=1, hot_temp=1.0, cold_temp=0.0,
+                   perturbation=0.02)
+    meta = dict(nx=nx, ny=ny, nz=nz, diameter=d, ellipsoid=ellip, re=re, re_eff=3.0 * u0 * d / (tau - 0.5),
+                tau=tau, thermal=thermal, rayleigh=ra, cells=nx * ny * nz)
+    return cfg, meta
+
+
+C5_CV_HALF = 0.40
+
+
+def c5_requests(n=256, seed=5150, **kw) -> list[tuple[dict, dict]]:
+    """The served-mode stream: n seeded requests (PCG64(seed))."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    return [c5_request(rng, **kw) for _ in range(n)]

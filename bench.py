@@ -17,8 +17,10 @@ per step-kernel launch from two rocprofv3 counter passes (FETCH_SIZE, then
 WRITE_SIZE) over a short run of the same case, each scaled by what the same
 counter reads for a plain copy kernel of known size in the same process.
 `secondary` holds the other precisions and cases (33.5M cells FP32/FP16/BF16,
-the FP64 sphere, the coupled thermal lattice); `cpu_baseline` times the CPU
-restatement on the host cores for a bounded number of steps of the same case.
+the FP64 sphere, the coupled thermal lattice); `served` the served-mode
+stream (solves/s, p50/p99 latency, against one-at-a-time solves);
+`cpu_baseline` times the CPU restatement on the host cores for a bounded
+number of steps of the same case.
 """
 from __future__ import annotations
 
@@ -34,6 +36,8 @@ import sys
 import tempfile
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "aeros-engine_amd"))
@@ -97,6 +101,79 @@ def cpu_baseline(cfg, budget_s=12.0, max_steps=40):
     return dict(value=round(cells * n / dt / 1e6, 3), unit="MLUPS", cores=threads, kind="port",
                 sample=f"{n} steps of the same {L.dims[0]}x{L.dims[1]}x{L.dims[2]} case after 1 warm-up step, "
                        f"oracle (C++ restatement, OpenMP {threads} threads)")
+
+
+# ------------------------------------------------------------------ served mode
+def solve_alone(A, W, cfg, round_steps, tol, max_steps):
+    """One request on its own lattice: rounds of round_steps, drag after each, the
+    served retire rule (|dCd| < tol over a round, or max_steps)."""
+    L = A.Lattice(A.default_config(**cfg, precision="fp16"))
+    area = L.projected_area(0)
+    u0 = float(np.float32(np.sqrt(np.float32(cfg["U0"]) * np.float32(cfg["U0"]))))
+    q = 0.5 * u0 * u0 * float(area)
+    steps, cd, prev, rounds = 0, 0.0, None, 0
+    while True:
+        L.step(round_steps)
+        steps = L.scalars()["step"]   # the step counter, which a divergence reset moves back
+        rounds += 1
+        cd = L.cv_force(W.C5_CV_HALF, W.C5_CV_HALF, W.C5_CV_HALF)[0] / q
+        if rounds >= 2 and abs(cd - prev) < tol:
+            break
+        if steps >= max_steps:
+            break
+        prev = cd
+    L.close()
+    return cd, steps
+
+
+def served_leg(A, W, n=256, base_k=32, round_steps=200, tol=1e-4, max_steps=4000, seed=5150, load=4.0):
+    """256 seeded requests (32^3 .. 160x80x80, spheres / ellipsoids, 30 % thermal), FP16
+    storage.  One-at-a-time: the first base_k requests solved back to back on the plain
+    solver.  Served: all n arrive as a Poisson stream at `load` times the one-at-a-time
+    rate and run on the server; latency = completion - arrival."""
+    reqs = W.c5_requests(n, seed)
+    t0 = time.perf_counter()
+    alone = [solve_alone(A, W, cfg, round_steps, tol, max_steps) for cfg, _ in reqs[:base_k]]
+    t_alone = time.perf_counter() - t0
+    base_rate = base_k / t_alone
+    log(f"[served] one-at-a-time: {base_k} requests in {t_alone:.2f} s = {base_rate:.3f} solves/s")
+    lam = load * base_rate
+    rng = np.random.Generator(np.random.PCG64(seed + 1))
+    gaps = rng.exponential(1.0 / lam, size=n)
+    srv = A.Server(precision="fp16", round_steps=round_steps, cd_tolerance=tol, max_steps=max_steps,
+                   cv_half=W.C5_CV_HALF)
+    cfgs = [A.default_config(**cfg, precision="fp16") for cfg, _ in reqs]
+    ids = []
+    start = srv.clock()
+    t_next = start
+    for k in range(n):
+        t_next += gaps[k]
+        while srv.clock() < t_next:
+            time.sleep(min(0.002, max(0.0, t_next - srv.clock())))
+        ids.append(srv.submit(cfgs[k]))
+    if not srv.wait_all(600.0):
+        raise RuntimeError("served requests did not finish")
+    res = [srv.poll(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+    makespan = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    same = sum(1 for k in range(base_k) if np.float64(res[k]["cd"]).tobytes() == np.float64(alone[k][0]).tobytes()
+               and res[k]["steps"] == alone[k][1])
+    served_rate = n / makespan
+    log(f"[served] {n} requests at lambda {lam:.2f}/s: {served_rate:.3f} solves/s, p50 {np.percentile(lat, 50):.2f} s, "
+        f"p99 {np.percentile(lat, 99):.2f} s, peak resident {st['peak_resident']}")
+    return {"requests": n, "lambda_per_s": round(lam, 3), "solves_per_s": round(served_rate, 3),
+            "p50_s": round(float(np.percentile(lat, 50)), 3), "p99_s": round(float(np.percentile(lat, 99)), 3),
+            "one_at_a_time_solves_per_s": round(base_rate, 3), "one_at_a_time_sample": base_k,
+            "speedup_vs_one_at_a_time": round(served_rate / base_rate, 2),
+            "converged": sum(r["state_name"] == "done" for r in res),
+            "stopped_at_cap": sum(r["state_name"] == "limit" for r in res),
+            "failed": sum(r["state_name"] == "failed" for r in res),
+            "identical_to_alone": f"{same}/{base_k}",
+            "served_mlups": round(st["cell_steps"] / max(st["busy_seconds"], 1e-9) / 1e6, 1),
+            "peak_resident": st["peak_resident"], "peak_bytes": st["peak_bytes"],
+            "rule": f"retire at |dCd| < {tol} over a {round_steps}-step round or at {max_steps} steps; FP16 storage"}
 
 
 # ------------------------------------------------------------------ HBM traffic
@@ -174,6 +251,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true", help="skip the other precisions and cases")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
+    ap.add_argument("--no-served", action="store_true", help="skip the served-mode leg")
     ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -234,6 +312,8 @@ def main():
                             "c1": "96x48x48 FP64 BGK sphere D=12, U0 0.05, zero-gradient outflow",
                             "c3": "128x64x128 coupled D3Q19+D3Q7 FP32, Rayleigh-Benard Ra 2500 Pr 0.71"}
         out["secondary"] = sec
+    if not args.no_served:
+        out["served"] = served_leg(A, W)
     if not args.no_traffic:
         tr, err = measure_traffic()
         if tr:
