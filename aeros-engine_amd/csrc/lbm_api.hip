@@ -77,6 +77,17 @@ inline int grid_for(int64_t n, int block = 256) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
 }
 
+// Step-kernel blocks: each owns this many consecutive 256-cell rows.  Short
+// runs keep the blocks that are resident at one time on a compact window of
+// the lattice and make the end of the launch even.
+#ifndef AEROS_ROWS_PER_BLOCK
+#define AEROS_ROWS_PER_BLOCK 4
+#endif
+inline int step_blocks(int64_t n) {
+    const int64_t rows = (n + BLOCK - 1) / BLOCK;
+    return (int)std::max<int64_t>(1, (rows + AEROS_ROWS_PER_BLOCK - 1) / AEROS_ROWS_PER_BLOCK);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ engine
@@ -115,7 +126,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
     hipStream_t st;
     int64_t n = 0;
     int grid = 1;
-    DevBuf f[2], mask, solid_u8, u[2], fld[6], world[4], g[2], T[2], buoy[2], parts, status, prev_rho, scratch;
+    DevBuf f[2], mask, solid_u8, u[2], fld[6], world[4], g[2], T[2], buoy[2], parts, acc, status, prev_rho, scratch;
     bool pre = true;          // buffers hold pre-collision populations (after init / reset)
     int cur = 0;              // lbmCurrentStep (host mirror)
     int resets = 0;
@@ -239,15 +250,9 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             }
         }
         if ((rc = status.alloc(sizeof(Status)))) return rc;
-        int dev = 0, cus = 0;
-        HIP_TRY(hipGetDevice(&dev));
-        HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-        int per_cu = 0;
-        const StepParams<R> P = params();
-        HIP_TRY(occupancy(P, &per_cu));
-        const int64_t want = (n + BLOCK - 1) / BLOCK;
-        grid = (int)std::max<int64_t>(1, std::min<int64_t>(want, (int64_t)std::max(1, per_cu) * std::max(1, cus)));
-        if ((rc = parts.alloc((size_t)grid * sizeof(Partial)))) return rc;
+        grid = step_blocks(n);
+        if ((rc = parts.alloc((size_t)2 * grid * sizeof(Partial)))) return rc;
+        if ((rc = acc.alloc((size_t)kAccSets * kAccSet * sizeof(int64_t)))) return rc;
         int64_t stride = 1;
         while (n / stride > 262144) stride *= 2;
         if ((rc = prev_rho.alloc((size_t)((n + stride - 1) / stride) * sizeof(float)))) return rc;
@@ -255,17 +260,6 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         return AEROS_OK;
     }
 
-    hipError_t occupancy(const StepParams<R>& P, int* per_cu) {
-        const bool force = P.fx != R(0) || P.fy != R(0) || P.fz != R(0);
-#define OCC(L, TH, F) hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, (const void*)&k_step<R, S, U, L, TH, F>, BLOCK, 0)
-        if (P.les) {
-            if (P.thermal) return force ? OCC(true, true, true) : OCC(true, true, false);
-            return force ? OCC(true, false, true) : OCC(true, false, false);
-        }
-        if (P.thermal) return force ? OCC(false, true, true) : OCC(false, true, false);
-        return force ? OCC(false, false, true) : OCC(false, false, false);
-#undef OCC
-    }
 
     int upload_mask() {
         const size_t NS = (size_t)n;
@@ -321,6 +315,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             for (int b = 0; b < 2; ++b) if ((rc = fill<R>(buoy[b], n, R(0)))) return rc;
             if ((rc = upload_T(Th.data(), true))) return rc;
         }
+        if ((rc = fill<int64_t>(acc, (int64_t)kAccSets * kAccSet, 0))) return rc;
         Status z{};
         host_status = z;
         HIP_TRY(hipMemcpyAsync(status.p, &z, sizeof z, hipMemcpyHostToDevice, st));
@@ -393,6 +388,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             B.t_out = T[p ^ 1].p;
         }
         B.parts = parts.as<Partial>();
+        B.acc = acc.as<int64_t>();
         B.status = status.as<Status>();
         (void)fields;
         return B;
@@ -413,10 +409,17 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             P.first = (was_pre && i == 0) ? 1 : 0;
             P.write_fields = (i == nsteps - 1) ? 1 : 0;
             P.dry = 0;
+            P.k = k;
+            P.pending = i > 0 ? 1 : 0;
             const StepBuffers B = buffers(k, P.first != 0, P.write_fields != 0);
             if (timing) HIP_TRY(hipEventRecord(events[(size_t)i * 2], st));
             HIP_TRY((launch_step<R, S, U>(P, B, grid, st)));
             if (timing) HIP_TRY(hipEventRecord(events[(size_t)i * 2 + 1], st));
+        }
+        {   // fold the call's last step into the status
+            const int k = start + nsteps - 1;
+            hipLaunchKernelGGL((k_fold<R>), dim3(1), dim3(BLOCK), 0, st, P, buffers(k, false, false), k, (uint32_t)grid);
+            HIP_TRY(hipGetLastError());
         }
         int rc = finish(start, nsteps, was_pre);
         if (rc) return rc;
@@ -455,6 +458,8 @@ template <class R, class S, class U> struct Engine final : EngineBase {
                 P.first = (was_pre && i == 0) ? 1 : 0;
                 P.write_fields = 1;
                 P.dry = 1;
+                P.k = sidx - 1;
+                P.pending = 0;
                 const StepBuffers B = buffers(sidx - 1, P.first != 0, true);
                 HIP_TRY((launch_step<R, S, U>(P, B, grid, st)));
             }
@@ -465,6 +470,8 @@ template <class R, class S, class U> struct Engine final : EngineBase {
                                mask.as<uint32_t>(), n, P, s.cfg.use_les ? u[p].as<U>() : nullptr,
                                fld[0].as<R>(), fld[1].as<R>(), fld[2].as<R>(), fld[3].as<R>());
             HIP_TRY(hipGetLastError());
+            int rc2 = fill<int64_t>(acc, (int64_t)kAccSets * kAccSet, 0);   // step indices restart from cur
+            if (rc2) return rc2;
             pre = true;
             resets++;
             S_.abort = 0;
@@ -472,7 +479,6 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             S_.avg_rho = 1.0f;
             const float a = s.inlet[0], b = s.inlet[1], c = s.inlet[2];
             S_.avg_ke = (float)((R)a * (R)a + (R)b * (R)b + (R)c * (R)c);
-            S_.ticket = 0;
             HIP_TRY(hipMemcpyAsync(status.p, &S_, sizeof S_, hipMemcpyHostToDevice, st));
         }
         host_status = S_;

@@ -78,6 +78,8 @@ template <class R> struct StepParams {
     int first;               // state holds pre-collision populations: collide only
     int write_fields;        // write rho/u/tke/strain fields this step
     int dry;                 // re-run of an aborted step: no abort check, no reduction
+    int k;                   // index of this step since the lattice was (re)initialised
+    int pending;             // step k - 1 is still to be folded into the status
     R tau0, cs, vel_scale2;  // sanitised tau, Smagorinsky constant, (flowSpeed/U0)^2
     R inv_tau0, s_minus0;    // 1/tau0 and the TRT s- at tau0 (exactly what the kernel would compute)
     R vd[7];                 // Van Driest table

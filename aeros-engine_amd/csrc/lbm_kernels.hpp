@@ -66,18 +66,24 @@ template <class T> __device__ inline gref<T> at(T* base, uint32_t i) {
     return *(AEROS_GLOBAL T*)(b + i * (uint32_t)sizeof(T));
 }
 
+// Population loads and stores.  Stores are non-temporal: a lattice written in
+// one step is read back only in the next, after the rest of the lattice has
+// streamed through the caches, so the stores should not displace lines that
+// are still to be read (measured 4-6 % faster steps).
+template <class T> __device__ inline void st_nt(AEROS_GLOBAL T& dst, T v) { __builtin_nontemporal_store(v, &dst); }
+
 template <class S, class R> struct Pop;
 template <> struct Pop<float, float> {
     __device__ static float ld(const float* p, uint32_t i, float) { return at(p, i); }
-    __device__ static void st(float* p, uint32_t i, float v, float) { at(p, i) = v; }
+    __device__ static void st(float* p, uint32_t i, float v, float) { st_nt(at(p, i), v); }
 };
 template <> struct Pop<double, double> {
     __device__ static double ld(const double* p, uint32_t i, double) { return at(p, i); }
-    __device__ static void st(double* p, uint32_t i, double v, double) { at(p, i) = v; }
+    __device__ static void st(double* p, uint32_t i, double v, double) { st_nt(at(p, i), v); }
 };
 template <> struct Pop<_Float16, float> {   // deviation from w_q, IEEE half, round to nearest even
     __device__ static float ld(const _Float16* p, uint32_t i, float w) { return w + (float)at(p, i); }
-    __device__ static void st(_Float16* p, uint32_t i, float v, float w) { at(p, i) = (_Float16)(v - w); }
+    __device__ static void st(_Float16* p, uint32_t i, float v, float w) { st_nt(at(p, i), (_Float16)(v - w)); }
 };
 __device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ inline uint16_t f_to_bf16(float x) {   // round to nearest even
@@ -88,7 +94,7 @@ __device__ inline uint16_t f_to_bf16(float x) {   // round to nearest even
 }
 template <> struct Pop<bf16_t, float> {     // deviation from w_q, bfloat16
     __device__ static float ld(const bf16_t* p, uint32_t i, float w) { return w + bf16_to_f(at(p, i).bits); }
-    __device__ static void st(bf16_t* p, uint32_t i, float v, float w) { at(p, i).bits = f_to_bf16(v - w); }
+    __device__ static void st(bf16_t* p, uint32_t i, float v, float w) { st_nt(at(p, i).bits, f_to_bf16(v - w)); }
 };
 
 // Population q of cell idx in a structure-of-arrays lattice of n cells, by
@@ -177,15 +183,29 @@ template <class R, int q> __device__ inline R guo(R ex, R ey, R ez, R Fx, R Fy, 
 }
 
 // ------------------------------------------------------------ reductions
+// The diagnostics of a step are reduced without any cross-block fence:
+//  * every block writes one Partial (plain stores) to the step's half of a
+//    double-buffered partials array, and adds its rho sum and NaN count into
+//    the step's set of decision accumulators (int64 atomics spread over
+//    kAccSlots words, so the result does not depend on their order);
+//  * the next launch (each block, from the accumulators) decides whether the
+//    step broke the loop (NaN storm / divergence), and its block 0 folds the
+//    partials in block order into the device Status; after the last step of a
+//    call a one-block kernel does the same.
+// Kernel boundaries order these accesses, so no fence is needed.
 struct Partial {
-    double rho, ke, tke;
+    int64_t rho_fx;          // sum of rho in 2^-32 fixed point (exact for float rho in [0.5, 1.5])
+    double ke, tke;
     float maxv;
     int32_t nan, neg, bad, rho_out, rho_nf, ulim;
 };
 
-// Device-resident scalar state, updated by the last block of every step
-// kernel exactly as the reference updates its globals
-// (aeros/src/lbm.cpp:1313-1319, 1554-1580).
+constexpr int kAccSlots = 16;              // atomic words per accumulated quantity
+constexpr int kAccSet = 2 * kAccSlots;     // rho_fx[kAccSlots], nan[kAccSlots]
+constexpr int kAccSets = 3;                // step k adds into set k % 3, reads set (k - 1) % 3, clears (k + 1) % 3
+
+// Device-resident scalar state, updated as the reference updates its globals
+// at the end of every step (aeros/src/lbm.cpp:1313-1319, 1554-1580).
 struct Status {
     int32_t step;            // lbmCurrentStep
     int32_t abort;           // 0, 1 = NaN storm (step not counted), 2 = divergence (step counted)
@@ -194,11 +214,51 @@ struct Status {
     float avg_rho, avg_ke, max_u_lb, max_u_world, tke, convergence, prev_ke;
     int32_t pad0;
     int64_t neg, bad, rho_out, rho_nf, ulim;
-    uint32_t ticket;         // last-block counter
-    int32_t pad1;
 };
 
 constexpr int BLOCK = 256;
+constexpr int WAVES = BLOCK / 64;
+
+// rho in 2^-32 fixed point.  A float in [0.5, 1.5] is a multiple of 2^-24,
+// so the conversion is exact and any summation order gives the same integer.
+__device__ inline int64_t rho_fixed(float r) {
+    const uint32_t u = __float_as_uint(r);
+    const uint64_t m = (u & 0x7fffffu) | 0x800000u;      // 24-bit significand
+    const int e = (int)((u >> 23) & 0xffu) - 127;        // -1 or 0 for [0.5, 2)
+    return (int64_t)(m << (9 + e));                      // m 2^(e-23) 2^32
+}
+__device__ inline int64_t rho_fixed(double r) { return (int64_t)__builtin_rint(r * 4294967296.0); }
+
+// Whether step k broke the loop, from its decision accumulators: 1 = NaN
+// storm, 2 = divergence of the fluid-mean density, else 0.
+template <class R>
+__device__ inline int step_decision(const StepParams<R>& P, int64_t rho_fx, int64_t nan, float* avg_rho) {
+    const float a = (float)((double)rho_fx * 0x1p-32 / (double)P.fluid_cells);
+    if (avg_rho) *avg_rho = a;
+    if (nan > P.nan_limit) return 1;
+    if (!__builtin_isfinite(a) || a > 1.2f || a < 0.8f) return 2;
+    return 0;
+}
+
+// Totals of one accumulator set (uniform loads, summed in a fixed order).
+__device__ inline void acc_totals(const int64_t* set, int64_t& rho_fx, int64_t& nan) {
+    int64_t a = 0, b = 0;
+#pragma unroll
+    for (int i = 0; i < kAccSlots; ++i) { a += set[i]; b += set[kAccSlots + i]; }
+    rho_fx = a;
+    nan = b;
+}
+
+template <class T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+    return v;
+}
 
 template <class T> __device__ inline void lds_tree_sum(T* s, int tid) {
     for (int off = BLOCK / 2; off > 0; off >>= 1) {
@@ -207,91 +267,108 @@ template <class T> __device__ inline void lds_tree_sum(T* s, int tid) {
     }
     __syncthreads();
 }
-__device__ inline void lds_tree_max(float* s, int tid) {
-    for (int off = BLOCK / 2; off > 0; off >>= 1) {
-        __syncthreads();
-        if (tid < off) s[tid] = fmaxf(s[tid], s[tid + off]);
-    }
-    __syncthreads();
-}
 
 struct RedLds {
-    double d[3][BLOCK];
-    float m[BLOCK];
-    int32_t i[6][BLOCK];
+    Partial w[WAVES];
+    Partial fold[BLOCK];
+    int flag;
 };
 
-__device__ inline void block_reduce(RedLds& L, int tid, Partial& p) {
-    L.d[0][tid] = p.rho; L.d[1][tid] = p.ke; L.d[2][tid] = p.tke; L.m[tid] = p.maxv;
-    L.i[0][tid] = p.nan; L.i[1][tid] = p.neg; L.i[2][tid] = p.bad; L.i[3][tid] = p.rho_out;
-    L.i[4][tid] = p.rho_nf; L.i[5][tid] = p.ulim;
-    for (int k = 0; k < 3; ++k) lds_tree_sum(L.d[k], tid);
-    lds_tree_max(L.m, tid);
-    for (int k = 0; k < 6; ++k) lds_tree_sum(L.i[k], tid);
-    p.rho = L.d[0][0]; p.ke = L.d[1][0]; p.tke = L.d[2][0]; p.maxv = L.m[0];
-    p.nan = L.i[0][0]; p.neg = L.i[1][0]; p.bad = L.i[2][0]; p.rho_out = L.i[3][0];
-    p.rho_nf = L.i[4][0]; p.ulim = L.i[5][0];
+// Block sum of the per-thread partials: xor butterflies inside each wave
+// (every lane ends with the same value), then the waves in index order.
+// The counters are almost always zero and are reduced only when some lane
+// of the wave has one.
+__device__ inline Partial block_reduce(RedLds& L, int tid, const Partial& p) {
+    Partial r;
+    r.rho_fx = wave_sum(p.rho_fx);
+    r.ke = wave_sum(p.ke);
+    r.tke = wave_sum(p.tke);
+    r.maxv = wave_max(p.maxv);
+    const bool any = __ballot((p.nan | p.neg | p.bad | p.rho_out | p.rho_nf | p.ulim) != 0) != 0;
+    if (any) {
+        r.nan = wave_sum(p.nan); r.neg = wave_sum(p.neg); r.bad = wave_sum(p.bad);
+        r.rho_out = wave_sum(p.rho_out); r.rho_nf = wave_sum(p.rho_nf); r.ulim = wave_sum(p.ulim);
+    } else {
+        r.nan = r.neg = r.bad = r.rho_out = r.rho_nf = r.ulim = 0;
+    }
+    const int wid = tid >> 6;
+    if ((tid & 63) == 0) L.w[wid] = r;
+    __syncthreads();
+    Partial t = L.w[0];
+#pragma unroll
+    for (int k = 1; k < WAVES; ++k) {
+        const Partial& q = L.w[k];
+        t.rho_fx += q.rho_fx; t.ke += q.ke; t.tke += q.tke; t.maxv = fmaxf(t.maxv, q.maxv);
+        t.nan += q.nan; t.neg += q.neg; t.bad += q.bad; t.rho_out += q.rho_out; t.rho_nf += q.rho_nf;
+        t.ulim += q.ulim;
+    }
+    return t;
 }
 
-// The last block to finish folds the per-block partials in block order and
-// applies the reference's end-of-step logic.
+// Fold the nb partials of one step in index order (fixed assignment of
+// partials to threads, then a fixed tree) and apply the reference's
+// end-of-step logic to the Status (aeros/src/lbm.cpp:1313-1319, 1554-1580).
+// `dec` is the step's decision from its accumulators, which every block of
+// the following launch computes identically.
 template <class R>
-__device__ __forceinline__ void finalize_step(const StepParams<R>& P, Partial* parts, Status* st, RedLds& L, int tid,
-                                              int nb) {
-    double a = 0, b = 0, c = 0;
+__device__ void fold_step(const StepParams<R>& P, const Partial* parts, int nb, Status* st, RedLds& L, int tid,
+                          int dec, float avg_rho) {
+    double ke = 0, tke = 0;
     float m = 0.0f;
-    int64_t n0 = 0, n1 = 0, n2 = 0, n3 = 0, n4 = 0, n5 = 0;
-    for (int k = tid; k < nb; k += BLOCK) {   // fixed assignment of partials to threads
+    int64_t c[5] = {0, 0, 0, 0, 0};
+    for (int k = tid; k < nb; k += BLOCK) {
         const Partial& q = parts[k];
-        a += q.rho; b += q.ke; c += q.tke; m = fmaxf(m, q.maxv);
-        n0 += q.nan; n1 += q.neg; n2 += q.bad; n3 += q.rho_out; n4 += q.rho_nf; n5 += q.ulim;
+        ke += q.ke; tke += q.tke; m = fmaxf(m, q.maxv);
+        c[0] += q.neg; c[1] += q.bad; c[2] += q.rho_out; c[3] += q.rho_nf; c[4] += q.ulim;
     }
-    // int64 counters: reuse the double slots bitwise-free via two passes
-    L.d[0][tid] = a; L.d[1][tid] = b; L.d[2][tid] = c; L.m[tid] = m;
-    for (int k = 0; k < 3; ++k) lds_tree_sum(L.d[k], tid);
-    lds_tree_max(L.m, tid);
-    const double sr = L.d[0][0], sk = L.d[1][0], stk = L.d[2][0];
-    const float mv = L.m[0];
+    // the int64 counters travel in the fixed-point slot pairs of the LDS partials
+    Partial& mine = L.fold[tid];
+    mine.ke = ke; mine.tke = tke; mine.maxv = m; mine.rho_fx = c[0];
     __syncthreads();
-    int64_t* li = reinterpret_cast<int64_t*>(&L.d[0][0]);   // 3*BLOCK doubles = room for 6 x BLOCK/2... use two rounds
-    int64_t cnt[6] = {n0, n1, n2, n3, n4, n5};
-    int64_t tot[6];
-    for (int r = 0; r < 2; ++r) {
-        for (int k = 0; k < 3; ++k) li[k * BLOCK + tid] = cnt[r * 3 + k];
-        for (int k = 0; k < 3; ++k) {
-            int64_t* s = li + k * BLOCK;
-            for (int off = BLOCK / 2; off > 0; off >>= 1) {
-                __syncthreads();
-                if (tid < off) s[tid] += s[tid + off];
-            }
+    for (int off = BLOCK / 2; off > 0; off >>= 1) {
+        if (tid < off) {
+            Partial& a = L.fold[tid];
+            const Partial& b = L.fold[tid + off];
+            a.ke += b.ke; a.tke += b.tke; a.maxv = fmaxf(a.maxv, b.maxv); a.rho_fx += b.rho_fx;
         }
         __syncthreads();
-        for (int k = 0; k < 3; ++k) tot[r * 3 + k] = li[k * BLOCK];
+    }
+    const double ske = L.fold[0].ke, stk = L.fold[0].tke;
+    const float mv = L.fold[0].maxv;
+    int64_t tot[5];
+    tot[0] = L.fold[0].rho_fx;
+    __syncthreads();
+    for (int r = 1; r < 5; ++r) {
+        L.fold[tid].rho_fx = c[r];
+        __syncthreads();
+        for (int off = BLOCK / 2; off > 0; off >>= 1) {
+            if (tid < off) L.fold[tid].rho_fx += L.fold[tid + off].rho_fx;
+            __syncthreads();
+        }
+        tot[r] = L.fold[0].rho_fx;
         __syncthreads();
     }
     if (tid == 0) {
-        if (tot[0] > P.nan_limit) {           // NaN storm: break before the stream
+        if (dec == 1) {                      // NaN storm: break before the stream
             st->abort = 1;
             st->abort_step = st->step + 1;
         } else {
             st->step += 1;
-            const float avg_rho = (float)(sr / (double)P.fluid_cells);
-            const float avg_ke = (float)(sk / (double)P.fluid_cells);
+            const float avg_ke = (float)(ske / (double)P.fluid_cells);
             st->avg_rho = avg_rho;
             st->avg_ke = avg_ke;
             st->max_u_lb = mv;
             st->tke = (float)(stk / (double)P.fluid_cells);
             st->max_u_world = mv * P.vel_scale;
-            st->neg = tot[1]; st->bad = tot[2]; st->rho_out = tot[3]; st->rho_nf = tot[4]; st->ulim = tot[5];
+            st->neg = tot[0]; st->bad = tot[1]; st->rho_out = tot[2]; st->rho_nf = tot[3]; st->ulim = tot[4];
             st->convergence = fabsf(avg_ke - st->prev_ke);
             st->prev_ke = avg_ke;
             st->converged = st->convergence < P.conv_threshold;
-            if (!__builtin_isfinite(avg_rho) || avg_rho > 1.2f || avg_rho < 0.8f) {
+            if (dec == 2) {
                 st->abort = 2;
                 st->abort_step = st->step;
             }
         }
-        st->ticket = 0;
     }
 }
 
@@ -308,7 +385,8 @@ struct StepBuffers {
     const void* b_in;          // buoyancy from the previous thermal step
     void* b_out;
     void* t_out;               // temperature
-    Partial* parts;
+    Partial* parts;            // 2 x nblk: the partials of steps of even / odd index
+    int64_t* acc;              // kAccSets x kAccSet decision accumulators
     Status* status;
 };
 
@@ -319,18 +397,34 @@ template <> struct VelAos<_Float16> { static constexpr int stride = 4; };
 
 // Flags of one launch: the state holds pre-collision populations (collide
 // only), write the output fields, re-run of an aborted step (no abort check,
-// no reduction).
+// no reduction), the step's index k since the lattice was (re)initialised
+// (selects the partials half and the accumulator sets), and whether step
+// k - 1 is still to be folded (every launch of a call but the first).
 struct StepFlags {
-    int first, write_fields, dry;
+    int first, write_fields, dry, k, pending;
 };
 
 // One lattice step over blocks blk of nblk (the whole grid for a single
 // lattice, the lattice's share of the launch in served mode).
 template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
 __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuffers& B, const StepFlags F,
-                                          const uint32_t blk, const uint32_t nblk, RedLds& L, int& is_last) {
+                                          const uint32_t blk, const uint32_t nblk, RedLds& L) {
     Status* st = B.status;
-    if (!F.dry && st->abort) return;   // an earlier step of this call broke the loop
+    if (!F.dry) {
+        if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;   // an earlier step broke the loop
+        if (F.pending) {
+            // did step k - 1 break the loop?  Every block decides the same from
+            // the accumulators; block 0 also folds that step's partials.
+            int64_t rfx, nan;
+            acc_totals(B.acc + (size_t)((F.k + kAccSets - 1) % kAccSets) * kAccSet, rfx, nan);
+            float avg_rho;
+            const int dec = step_decision(P, rfx, nan, &avg_rho);
+            if (blk == 0)
+                fold_step<R>(P, B.parts + (size_t)((F.k - 1) & 1) * nblk, (int)nblk, st, L, threadIdx.x, dec, avg_rho);
+            if (dec) return;
+        }
+        if (blk == 0 && threadIdx.x < kAccSet) B.acc[(size_t)((F.k + 1) % kAccSets) * kAccSet + threadIdx.x] = 0;
+    }
 
     const S* fin_ = static_cast<const S*>(B.f_in);
     const uint32_t n = (uint32_t)P.n;
@@ -340,18 +434,21 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
     const int tid = threadIdx.x;
     constexpr int US = VelAos<U>::stride;
 
-    Partial acc{0.0, 0.0, 0.0, 0.0f, 0, 0, 0, 0, 0, 0};
+    Partial acc{0, 0.0, 0.0, 0.0f, 0, 0, 0, 0, 0, 0};
     R maxu2 = R(0);
 
-    // Each iteration issues the next cell's mask load first, then the
-    // Smagorinsky stencil and the populations back to back, so a cell costs
-    // one memory round trip.
-    const uint32_t stride = nblk * BLOCK;
-    uint32_t cell = blk * BLOCK + tid;
-    uint32_t m_next = cell < n ? at(B.mask, cell) : 0u;
-    for (; cell < n; cell += stride) {
+    // The block owns one contiguous run of cells, `per` = a whole number of
+    // 256-cell rows, and walks it a row at a time.  Each iteration issues the
+    // next row's mask load first, then the Smagorinsky stencil and the
+    // populations back to back, so a cell costs one memory round trip.
+    const uint32_t per = ((n + nblk * BLOCK - 1) / (nblk * BLOCK)) * BLOCK;
+    const uint32_t cbeg = blk * per;
+    const uint32_t cend = cbeg + per < n ? cbeg + per : n;
+    uint32_t cell = cbeg + tid;
+    uint32_t m_next = cell < cend ? at(B.mask, cell) : 0u;
+    for (; cell < cend; cell += BLOCK) {
         const uint32_t m = m_next;
-        if (cell + stride < n) m_next = at(B.mask, cell + stride);
+        if (cell + BLOCK < cend) m_next = at(B.mask, cell + BLOCK);
         if (m & M_SOLID) continue;   // solid/ground: fields are constant (rho 1, u 0), never stored
         const bool inlet = m & M_INLET, outlet = m & M_OUTLET;
 
@@ -646,7 +743,7 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
             at(static_cast<R*>(B.fld[5]), cell) = smag;
         }
         maxu2 = usqr > maxu2 ? usqr : maxu2;   // max |u|^2; the sqrt is taken once per thread
-        acc.rho += (double)rho;
+        acc.rho_fx += rho_fixed(rho);
         acc.ke += (double)usqr;
         acc.tke += (double)tk;
         acc.nan += nanc; acc.bad += badc;
@@ -716,31 +813,43 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
         }
     }
 
-    // ---------------- fixed-order reduction and end-of-step logic
+    // ---------------- this step's partial and decision accumulators
     if (F.dry) return;
     // max sqrt(usqr) == sqrt(max usqr): correctly rounded sqrt and the cast
     // to float are monotone, so the block maximum is unchanged.
     acc.maxv = (float)__builtin_sqrt(maxu2);
-    block_reduce(L, tid, acc);
+    const Partial bp = block_reduce(L, tid, acc);
     if (tid == 0) {
-        B.parts[blk] = acc;
-        __threadfence();
-        const uint32_t t = atomicAdd(&st->ticket, 1u);
-        is_last = (t == nblk - 1);
+        B.parts[(size_t)(F.k & 1) * nblk + blk] = bp;
+        int64_t* set = B.acc + (size_t)(F.k % kAccSets) * kAccSet;
+        atomicAdd((unsigned long long*)&set[blk % kAccSlots], (unsigned long long)bp.rho_fx);
+        if (bp.nan) atomicAdd((unsigned long long*)&set[kAccSlots + blk % kAccSlots], (unsigned long long)(int64_t)bp.nan);
     }
-    __syncthreads();
-    if (is_last) {
-        __threadfence();
-        finalize_step<R>(P, B.parts, st, L, tid, (int)nblk);
-    }
+}
+
+// After the last launch of a call: fold that step (index k) as the next
+// launch would.  One block.
+template <class R>
+__device__ __forceinline__ void fold_last(const StepParams<R>& P, const StepBuffers& B, int k, uint32_t nblk, RedLds& L) {
+    Status* st = B.status;
+    if (__hip_atomic_load(&st->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    int64_t rfx, nan;
+    acc_totals(B.acc + (size_t)(k % kAccSets) * kAccSet, rfx, nan);
+    float avg_rho;
+    const int dec = step_decision(P, rfx, nan, &avg_rho);
+    fold_step<R>(P, B.parts + (size_t)(k & 1) * nblk, (int)nblk, st, L, threadIdx.x, dec, avg_rho);
+}
+template <class R>
+__global__ __launch_bounds__(BLOCK) void k_fold(StepParams<R> P, StepBuffers B, int k, uint32_t nblk) {
+    __shared__ RedLds L;
+    fold_last<R>(P, B, k, nblk, L);
 }
 
 template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
 __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) {
     __shared__ RedLds L;
-    __shared__ int is_last;
-    step_grid<R, S, U, LES, THERM, FORCE>(P, B, StepFlags{P.first, P.write_fields, P.dry}, blockIdx.x, gridDim.x, L,
-                                          is_last);
+    step_grid<R, S, U, LES, THERM, FORCE>(P, B, StepFlags{P.first, P.write_fields, P.dry, P.k, P.pending}, blockIdx.x,
+                                          gridDim.x, L);
 }
 
 // ------------------------------------------------------------ served mode
@@ -764,14 +873,21 @@ template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
 __global__ __launch_bounds__(BLOCK) void k_step_batch(const BatchCase<R>* cases, const BlockMap* map, int i,
                                                       int nsteps) {
     __shared__ RedLds L;
-    __shared__ int is_last;
     const BlockMap bm = map[blockIdx.x];
     const BatchCase<R>& C = cases[bm.ci];
     const int k = C.start + i;
-    const StepFlags F{(C.pre && i == 0) ? 1 : 0, (i == nsteps - 1) ? 1 : 0, 0};
+    const StepFlags F{(C.pre && i == 0) ? 1 : 0, (i == nsteps - 1) ? 1 : 0, 0, k, i > 0 ? 1 : 0};
     StepBuffers B = C.buf[k & 1];
     if (F.first) B.f_in = B.f_out;
-    step_grid<R, S, U, LES, THERM, FORCE>(C.P, B, F, bm.blk, C.nblocks, L, is_last);
+    step_grid<R, S, U, LES, THERM, FORCE>(C.P, B, F, bm.blk, C.nblocks, L);
+}
+// The end of a served round: one block per case folds its last step.
+template <class R>
+__global__ __launch_bounds__(BLOCK) void k_fold_batch(const BatchCase<R>* cases, int nsteps) {
+    __shared__ RedLds L;
+    const BatchCase<R>& C = cases[blockIdx.x];
+    const int k = C.start + nsteps - 1;
+    fold_last<R>(C.P, C.buf[k & 1], k, C.nblocks, L);
 }
 
 // ------------------------------------------------------------ other kernels
