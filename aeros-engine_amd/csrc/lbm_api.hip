@@ -77,15 +77,16 @@ inline int grid_for(int64_t n, int block = 256) {
     return (int)std::max<int64_t>(1, std::min<int64_t>(g, 65536));
 }
 
-// Step-kernel blocks: each owns this many consecutive 256-cell rows.  Short
+// Step-kernel blocks: each owns a run of consecutive 256-cell rows.  Short
 // runs keep the blocks that are resident at one time on a compact window of
-// the lattice and make the end of the launch even.
-#ifndef AEROS_ROWS_PER_BLOCK
-#define AEROS_ROWS_PER_BLOCK 4
-#endif
+// the lattice and make the end of the launch even; longer runs amortise the
+// block's reduction.  Measured best: 4 rows from 8M cells up (33.5M cells:
+// 4 rows 1.19 ms, 2 rows 1.21, 8 rows 1.25), 2 rows around 1-8M cells
+// (3.41M cells: 2 rows 0.136 ms, 4 rows 0.149), 1 row below.
 inline int step_blocks(int64_t n) {
     const int64_t rows = (n + BLOCK - 1) / BLOCK;
-    return (int)std::max<int64_t>(1, (rows + AEROS_ROWS_PER_BLOCK - 1) / AEROS_ROWS_PER_BLOCK);
+    const int64_t run = rows >= 32768 ? 4 : (rows >= 4096 ? 2 : 1);
+    return (int)std::max<int64_t>(1, (rows + run - 1) / run);
 }
 
 }  // namespace
@@ -216,6 +217,11 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         P.t_lo = (R)(c.cold_temp - 1.0f);
         P.t_hot = (R)c.hot_temp; P.t_cold = (R)c.cold_temp;
         P.t_fixed = c.fixed_wall_temps != 0;
+        P.inlet_face = face_bit(s.axis, s.sign > 0 ? 0 : 1);
+        P.outlet_face = face_bit(s.axis, s.sign > 0 ? 1 : 0);
+        P.wrap_faces = (c.periodic_x ? (M_X0 | M_X1) : 0u) | (c.periodic_z ? (M_Z0 | M_Z1) : 0u);
+        P.inv_nx = 1.0 / (double)s.nx; P.half_inv_nx = 0.5 / (double)s.nx;
+        P.inv_ny = 1.0 / (double)s.ny; P.half_inv_ny = 0.5 / (double)s.ny;
         P.fluid_cells = s.fluid;
         P.nan_limit = n / 10;
         P.conv_threshold = c.convergence_threshold;

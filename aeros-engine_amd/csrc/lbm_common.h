@@ -19,19 +19,23 @@ constexpr int Q7 = 7;   // D3Q7 (first seven D3Q19 directions)
 //   bits 20..22 Van Driest table index: wall distance 0..5, 6 = no damping
 //   bit  23     Smagorinsky strain is defined here (interior, not on the inlet
 //               or outlet plane, no solid 6-neighbour)
-//   bit  24     inlet plane (geometric)      bit 25  outlet plane (geometric)
-//   bit  26     shell cell: the general pull (wrap / out-of-domain copy) applies
-//   bit  27     pull = inlet equilibrium     bit 28  pull = convective outlet
+//   bits 24..29 the cell lies on the domain face x = 0, x = nx-1, y = 0,
+//               y = ny-1, z = 0, z = nz-1 (any of them: a shell cell; the
+//               inlet and outlet planes are two of these faces)
+//   bit  30     pull = inlet equilibrium     bit 31  pull = convective outlet
 constexpr uint32_t M_SOLID = 1u << 19;
 constexpr int M_VD_SHIFT = 20;
 constexpr uint32_t M_VD_MASK = 7u << 20;
 constexpr uint32_t M_STRAIN = 1u << 23;
-constexpr uint32_t M_INLET = 1u << 24;
-constexpr uint32_t M_OUTLET = 1u << 25;
-constexpr uint32_t M_SHELL = 1u << 26;
-constexpr uint32_t M_PULL_INLET = 1u << 27;
-constexpr uint32_t M_PULL_OUTLET = 1u << 28;
+constexpr int M_FACE_SHIFT = 24;
+constexpr uint32_t M_X0 = 1u << 24, M_X1 = 1u << 25, M_Y0 = 1u << 26, M_Y1 = 1u << 27, M_Z0 = 1u << 28,
+                   M_Z1 = 1u << 29;
+constexpr uint32_t M_FACES = 63u << 24;
+constexpr uint32_t M_PULL_INLET = 1u << 30;
+constexpr uint32_t M_PULL_OUTLET = 1u << 31;
 constexpr uint32_t M_LINKS = (1u << 19) - 1u;
+// face bit of the plane where axis a (0..2) is 0 (hi = 0) or N-1 (hi = 1)
+constexpr uint32_t face_bit(int axis, int hi) { return 1u << (M_FACE_SHIFT + 2 * axis + hi); }
 
 // Result of the host-side part of initLBM (domain, mask, wall distance, tau).
 struct Setup {
@@ -92,6 +96,12 @@ template <class R> struct StepParams {
     R t_inv_tau, t_ref, t_g, t_hi, t_lo;
     R t_hot, t_cold;
     int t_fixed;
+    // domain faces: the inlet and outlet planes, and the faces a periodic
+    // axis wraps (their shell cells take the general pull)
+    uint32_t inlet_face, outlet_face, wrap_faces;
+    // cell -> (x, y, z): floor(c / d) = (uint)(c inv_d + half_inv_d) in FP64,
+    // exact for c < 2^26 and d < 2^25
+    double inv_nx, half_inv_nx, inv_ny, half_inv_ny;
     // reduction
     int64_t fluid_cells;
     int64_t nan_limit;       // total / 10

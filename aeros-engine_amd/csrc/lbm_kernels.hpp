@@ -131,6 +131,22 @@ template <class S> struct Lat {
 
 template <class R> __device__ inline bool fin(R x) { return __builtin_isfinite(x); }
 
+// Faces whose shell cells lose the source of direction q (source outside the
+// domain along that face's normal).
+__device__ __host__ constexpr uint32_t out_faces(int q) {
+    return (cx_of(q) > 0 ? M_X0 : 0u) | (cx_of(q) < 0 ? M_X1 : 0u) | (cy_of(q) > 0 ? M_Y0 : 0u) |
+           (cy_of(q) < 0 ? M_Y1 : 0u) | (cz_of(q) > 0 ? M_Z0 : 0u) | (cz_of(q) < 0 ? M_Z1 : 0u);
+}
+
+// Cell index -> coordinates without an integer division.
+template <class R> __device__ inline void cell_xyz(const StepParams<R>& P, uint32_t c, int& x, int& y, int& z) {
+    const uint32_t t = (uint32_t)__builtin_fma((double)c, P.inv_nx, P.half_inv_nx);   // c / nx
+    x = (int)(c - t * (uint32_t)P.nx);
+    const uint32_t zz = (uint32_t)__builtin_fma((double)t, P.inv_ny, P.half_inv_ny); // t / ny
+    y = (int)(t - zz * (uint32_t)P.ny);
+    z = (int)zz;
+}
+
 // Division, reciprocal and square root of the step kernel.  The FP32 and
 // FP64 paths round them correctly, as the reference does; the 16-bit
 // storage paths use the hardware's approximate reciprocal and square root
@@ -464,7 +480,7 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
         const uint32_t m = m_next;
         if (cell + BLOCK < cend) m_next = at(B.mask, cell + BLOCK);
         if (m & M_SOLID) continue;   // solid/ground: fields are constant (rho 1, u 0), never stored
-        const bool inlet = m & M_INLET, outlet = m & M_OUTLET;
+        const bool inlet = m & P.inlet_face, outlet = m & P.outlet_face;
 
         // ---------------- previous-step velocity around the cell for the
         // Smagorinsky strain (aeros/src/lbm.cpp:979-1036).  With 4- and 8-byte
@@ -501,7 +517,7 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
         // likewise the shifted temperature populations and the buoyancy of an
         // interior cell (solid sources are replaced after the collision)
         R gq[Q7], bq = R(0);
-        const bool early_g = THERM && EARLY_U && !(m & M_SHELL);
+        const bool early_g = THERM && EARLY_U && !(m & M_FACES);
         if (early_g) {
             const R* g = static_cast<const R*>(B.g_in);
 #pragma unroll
@@ -514,7 +530,7 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
         if (F.first) {
 #pragma unroll
             for (int q = 0; q < Q; ++q) f[q] = lin.ld(cell, (uint32_t)q * n, w_of<R>(q));
-        } else if (!(m & M_SHELL)) {
+        } else if (!(m & M_FACES)) {
             // interior: the source of q is cell - c_q; a uniform shifted base per direction
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
@@ -541,12 +557,26 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
                 const R fi2 = lin.ld(ci2, (uint32_t)q * n, w);
                 f[q] = fo + P.conv_c * (fi - fo) + P.conv_k2 * (fi - fi2);
             }
+        } else if (!(m & P.wrap_faces)) {
+            // domain shell with no periodic wrap: a source outside the domain
+            // is replaced by the cell's own population (aeros/src/lbm.cpp:1392-1394);
+            // in-domain sources pull (or bounce back) as in the interior
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int off = cx_of(q) + cy_of(q) * nx + cz_of(q) * (int)sz;
+                const uint32_t src = (m & out_faces(q)) ? cell : cell - (uint32_t)off;
+                f[q] = lin.ld(src, (uint32_t)q * n, w_of<R>(q));
+            }
+            if (m & M_LINKS) {
+#pragma unroll
+                for (int q = 1; q < Q; ++q)
+                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, w_of<R>(opp_of(q)));
+            }
         } else {
-            // domain shell: periodic wrap, out-of-domain sources copy the
-            // cell's own population (aeros/src/lbm.cpp:1362-1409)
-            const int x = (int)(cell % (uint32_t)nx);
-            const int y = (int)((cell / (uint32_t)nx) % (uint32_t)ny);
-            const int z = (int)(cell / sz);
+            // domain shell across a periodic face: wrap, out-of-domain sources
+            // copy the cell's own population (aeros/src/lbm.cpp:1362-1409)
+            int x, y, z;
+            cell_xyz(P, cell, x, y, z);
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 int sx = x - cx_of(q), syy = y - cy_of(q), szz = z - cz_of(q);
@@ -809,13 +839,15 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
             const R* g = static_cast<const R*>(B.g_in);
             R* go = static_cast<R*>(B.g_out);
             R gi[Q7];
-            if (!(m & M_SHELL)) {
+            if (!(m & M_FACES)) {
 #pragma unroll
                 for (int i = 0; i < Q7; ++i) {
                     const int off = cx_of(i) + cy_of(i) * nx + cz_of(i) * (int)sz;
                     if ((m >> i) & 1u) {
                         const R go_ = at(g + (size_t)opp_of(i) * n, cell);
-                        const int wy = (int)((cell / (uint32_t)nx) % (uint32_t)ny) - cy_of(i);
+                        int cx_, cy_, cz_;
+                        cell_xyz(P, cell, cx_, cy_, cz_);
+                        const int wy = cy_ - cy_of(i);
                         gi[i] = (P.t_fixed && (wy <= 0 || wy >= ny - 1))
                                     ? (-go_ + R(2) * w7_of<R>(i) * (wy <= 0 ? P.t_hot : P.t_cold)) : go_;
                     } else {
@@ -823,9 +855,8 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
                     }
                 }
             } else {
-                const int x = (int)(cell % (uint32_t)nx);
-                const int y = (int)((cell / (uint32_t)nx) % (uint32_t)ny);
-                const int z = (int)(cell / sz);
+                int x, y, z;
+                cell_xyz(P, cell, x, y, z);
 #pragma unroll
                 for (int i = 0; i < Q7; ++i) {
                     int sx = x - cx_of(i), syy = y - cy_of(i), szz = z - cz_of(i);
@@ -996,10 +1027,13 @@ __global__ void k_build_mask(uint32_t* mask, const uint8_t* solid, const uint8_t
         const int p[3] = {x, y, z}, N[3] = {nx, ny, nz};
         const bool inl = sign > 0 ? p[axis] == 0 : p[axis] == N[axis] - 1;
         const bool outl = sign > 0 ? p[axis] == N[axis] - 1 : p[axis] == 0;
-        if (inl) m |= M_INLET;
-        if (outl) m |= M_OUTLET;
-        const bool shell = x == 0 || x == nx - 1 || y == 0 || y == ny - 1 || z == 0 || z == nz - 1;
-        if (shell) m |= M_SHELL;
+        if (x == 0) m |= M_X0;
+        if (x == nx - 1) m |= M_X1;
+        if (y == 0) m |= M_Y0;
+        if (y == ny - 1) m |= M_Y1;
+        if (z == 0) m |= M_Z0;
+        if (z == nz - 1) m |= M_Z1;
+        const bool shell = (m & M_FACES) != 0;
         if (!sol) {
             if (inl && !periodic_x) m |= M_PULL_INLET;
             else if (outl && conv_outlet && !periodic_x) m |= M_PULL_OUTLET;
