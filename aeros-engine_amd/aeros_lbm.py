@@ -250,6 +250,8 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
         "aeros_lbm_equilibrium_moments": (C.c_int, [C.c_float] * 4 + [P(C.c_float)] * 4),
         "aeros_voxelize": (C.c_int, [P(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                      P(C.c_float), P(C.c_float), P(C.c_int32)]),
+        "aeros_voxelize_gpu": (C.c_int, [P(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_int32,
+                                         P(C.c_float), P(C.c_float), C.c_int32, P(C.c_int32)]),
         "aeros_voxel_plan": (C.c_int, [P(C.c_float), P(C.c_float), C.c_float, C.c_int32,
                                        P(C.c_int32), P(C.c_float), P(C.c_float)]),
         "aeros_lbm_last_error": (C.c_int, [C.c_char_p, C.c_int32]),
@@ -446,6 +448,19 @@ def voxelize(tris, dims, box_min, cell):
     cs = (C.c_float * 3)(*cell)
     _check(lib.aeros_voxelize(tris.ctypes.data_as(C.POINTER(C.c_float)), tris.shape[0], nx, ny, nz, mn, cs,
                               out.ctypes.data_as(C.POINTER(C.c_int32))), "aeros_voxelize")
+    return out.reshape(nz, ny, nx)
+
+
+def voxelize_gpu(tris, dims, box_min, cell, device: int = 0):
+    """The same vote on the device (bit-identical to voxelize); needs a gfx950 GPU."""
+    lib = load_library()
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    nx, ny, nz = dims
+    out = np.empty(nx * ny * nz, dtype=np.int32)
+    mn = (C.c_float * 3)(*box_min)
+    cs = (C.c_float * 3)(*cell)
+    _check(lib.aeros_voxelize_gpu(tris.ctypes.data_as(C.POINTER(C.c_float)), tris.shape[0], nx, ny, nz, mn, cs,
+                                  int(device), out.ctypes.data_as(C.POINTER(C.c_int32))), "aeros_voxelize_gpu")
     return out.reshape(nz, ny, nx)
 
 

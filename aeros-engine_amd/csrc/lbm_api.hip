@@ -940,6 +940,36 @@ int aeros_voxelize(const float* tris, int64_t ntris, int32_t nx, int32_t ny, int
     return voxelize_host(tris, ntris, nx, ny, nz, box_min, cell, out);
 }
 
+int aeros_voxelize_gpu(const float* tris, int64_t ntris, int32_t nx, int32_t ny, int32_t nz, const float box_min[3],
+                       const float cell[3], int32_t device, int32_t* out) {
+    if ((!tris && ntris > 0) || !box_min || !cell || !out || nx <= 0 || ny <= 0 || nz <= 0)
+        return set_err(AEROS_ERR_ARG, "bad voxelize arguments");
+    int rc = check_device(device);
+    if (rc) return rc;
+    const int64_t n = (int64_t)nx * ny * nz;
+    std::vector<VoxNode> nodes;
+    std::vector<VoxTri> vt;
+    voxel_bvh(tris, ntris, nodes, vt);
+    if (nodes.empty()) {   // no usable triangle: nothing is solid
+        std::fill(out, out + n, 0);
+        return AEROS_OK;
+    }
+    float jit[3];
+    voxel_jitter(cell, jit);
+    DevBuf dn, dt, dout;
+    if ((rc = dn.alloc(nodes.size() * sizeof(VoxNode))) || (rc = dt.alloc(vt.size() * sizeof(VoxTri))) ||
+        (rc = dout.alloc((size_t)n * sizeof(int32_t))))
+        return rc;
+    HIP_TRY(hipMemcpy(dn.p, nodes.data(), nodes.size() * sizeof(VoxNode), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(dt.p, vt.data(), vt.size() * sizeof(VoxTri), hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_voxelize, dim3(grid_for(n)), dim3(256), 0, 0, dn.as<VoxNode>(), dt.as<VoxTri>(), nx, ny, n,
+                       box_min[0], box_min[1], box_min[2], cell[0], cell[1], cell[2], jit[0], jit[1], jit[2],
+                       dout.as<int32_t>());
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpy(out, dout.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    return AEROS_OK;
+}
+
 // buildVoxelGrid's grid (aeros/src/voxel_grid.cpp:250-300, voxelPlan at 172-223)
 int aeros_voxel_plan(const float bb_min[3], const float bb_max[3], float max_dim, int32_t res, int32_t dims[3],
                      float box_min[3], float box_max[3]) {

@@ -73,6 +73,22 @@ void curl_noise_velocity(const Setup& s, std::vector<float>& u3);
 int voxelize_host(const float* tris, int64_t ntris, int nx, int ny, int nz, const float mn[3],
                   const float cs[3], int32_t* out);
 
+// The voxelizer's BVH flattened for the device: nodes in build order (node 0
+// is the root; a leaf has count > 0 and owns triangles [first, first+count)),
+// triangles in leaf order.
+struct VoxNode {
+    float lo[3], hi[3];
+    int32_t left, right, first, count;
+};
+struct VoxTri {
+    float a[3], b[3], c[3];
+};
+// Drops non-finite and degenerate triangles and builds the BVH exactly as
+// voxelize_host does.  Empty `nodes` means no usable triangle.
+void voxel_bvh(const float* tris, int64_t ntris, std::vector<VoxNode>& nodes, std::vector<VoxTri>& out_tris);
+// Ray jitter of the three-axis vote for a grid of cell size cs.
+void voxel_jitter(const float cs[3], float jit[3]);
+
 // ----------------------------------------------------------------- kernels
 template <class R> struct StepParams {
     int nx, ny, nz;

@@ -1345,6 +1345,86 @@ __global__ void k_thermal_diag(const R* T, const uint32_t* mask, int nx, int ny,
 
 __global__ void k_probe(int* d) { d[threadIdx.x] += 1; }
 
+// ------------------------------------------------------------ voxelizer
+// The three-ray majority vote of the set-mask producer (aeros/src/mesh_bvh.h:
+// 62-185, aeros/src/voxel_grid.cpp:337-351) on the device: one thread per
+// voxel casts its three jittered axis rays through the host-built BVH with
+// the same float slab test and double-precision Moller-Trumbore test as the
+// host voxelizer, so the mask is the same bit for bit.  Hits are counted
+// per triangle, so the traversal order does not matter.
+__device__ inline bool vox_box(const float o[3], const float inv[3], const VoxNode& nd) {
+    float t0 = -__builtin_inff(), t1 = __builtin_inff();
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float a = (nd.lo[k] - o[k]) * inv[k], b = (nd.hi[k] - o[k]) * inv[k];
+        if (a > b) { const float t = a; a = b; b = t; }
+        if (__builtin_isnan(a) || __builtin_isnan(b)) continue;   // ray in the slab plane
+        t0 = t0 < a ? a : t0;
+        t1 = b < t1 ? b : t1;
+        if (t1 < t0) return false;
+    }
+    return t1 >= 0.0f;
+}
+__device__ inline int vox_tri(const float o[3], const float d[3], const VoxTri& t) {
+    const double e1[3] = {(double)t.b[0] - t.a[0], (double)t.b[1] - t.a[1], (double)t.b[2] - t.a[2]};
+    const double e2[3] = {(double)t.c[0] - t.a[0], (double)t.c[1] - t.a[1], (double)t.c[2] - t.a[2]};
+    const double dd[3] = {d[0], d[1], d[2]};
+    const double p[3] = {dd[1] * e2[2] - dd[2] * e2[1], dd[2] * e2[0] - dd[0] * e2[2], dd[0] * e2[1] - dd[1] * e2[0]};
+    const double det = e1[0] * p[0] + e1[1] * p[1] + e1[2] * p[2];
+    const double scale = __builtin_fabs(e1[0] * e2[1] - e1[1] * e2[0]) + __builtin_fabs(e1[1] * e2[2] - e1[2] * e2[1]) +
+                         __builtin_fabs(e1[2] * e2[0] - e1[0] * e2[2]) + 1e-30;
+    if (__builtin_fabs(det) < 1e-12 * scale) return 0;
+    const double inv = 1.0 / det;
+    const double sv[3] = {(double)o[0] - t.a[0], (double)o[1] - t.a[1], (double)o[2] - t.a[2]};
+    const double u = (sv[0] * p[0] + sv[1] * p[1] + sv[2] * p[2]) * inv;
+    if (u < 0.0 || u > 1.0) return 0;
+    const double q[3] = {sv[1] * e1[2] - sv[2] * e1[1], sv[2] * e1[0] - sv[0] * e1[2], sv[0] * e1[1] - sv[1] * e1[0]};
+    const double v = (dd[0] * q[0] + dd[1] * q[1] + dd[2] * q[2]) * inv;
+    if (v < 0.0 || u + v > 1.0) return 0;
+    const double tt = (e2[0] * q[0] + e2[1] * q[1] + e2[2] * q[2]) * inv;
+    return (tt > 1e-7 && tt < 1e30) ? 1 : 0;
+}
+// Crossings of the ray o + t e_axis, t > 1e-7.
+__device__ inline int vox_crossings(const VoxNode* nodes, const VoxTri* tris, const float o[3], int axis) {
+    const float d[3] = {axis == 0 ? 1.0f : 0.0f, axis == 1 ? 1.0f : 0.0f, axis == 2 ? 1.0f : 0.0f};
+    const float inv[3] = {axis == 0 ? 1.0f : __builtin_inff(), axis == 1 ? 1.0f : __builtin_inff(),
+                          axis == 2 ? 1.0f : __builtin_inff()};
+    if (!vox_box(o, inv, nodes[0])) return 0;
+    constexpr int kStack = 64;   // the BVH is at most 48 levels deep, so at most 49 entries are pending
+    int stack[kStack];
+    int sp = 0, hits = 0;
+    stack[sp++] = 0;
+    while (sp) {
+        const VoxNode& nd = nodes[stack[--sp]];
+        if (nd.count) {
+            for (int i = nd.first; i < nd.first + nd.count; ++i) hits += vox_tri(o, d, tris[i]);
+        } else {
+            if (sp < kStack && vox_box(o, inv, nodes[nd.left])) stack[sp++] = nd.left;
+            if (sp < kStack && vox_box(o, inv, nodes[nd.right])) stack[sp++] = nd.right;
+        }
+    }
+    return hits;
+}
+__global__ __launch_bounds__(256) void k_voxelize(const VoxNode* nodes, const VoxTri* tris, int nx, int ny, int64_t n,
+                                                  float mnx, float mny, float mnz, float csx, float csy, float csz,
+                                                  float jx, float jy, float jz, int32_t* out) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int i = (int)(c % nx);
+        const int64_t r = c / nx;   // k ny + j
+        const int j = (int)(r % ny), k = (int)(r / ny);
+        const float pz = mnz + ((float)k + 0.5f) * csz, py = mny + ((float)j + 0.5f) * csy;
+        const float px = mnx + ((float)i + 0.5f) * csx;
+        const float ox[3] = {px, py + jy, pz + jz};
+        const float oy[3] = {px + jz, py, pz + jx};
+        const float oz[3] = {px + jy, py + jx, pz};
+        int votes = 0;
+        votes += vox_crossings(nodes, tris, ox, 0) & 1;
+        votes += vox_crossings(nodes, tris, oy, 1) & 1;
+        votes += vox_crossings(nodes, tris, oz, 2) & 1;
+        out[c] = votes >= 2 ? 1 : 0;
+    }
+}
+
 // Fill a typed buffer with a value (graph-capturable, unlike hipMemset).
 template <class T> __global__ void k_fill(T* p, int64_t n, T v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;

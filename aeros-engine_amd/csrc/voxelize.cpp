@@ -38,6 +38,21 @@ public:
     }
     bool empty() const { return nodes_.empty(); }
 
+    void flatten(std::vector<VoxNode>& nodes, std::vector<VoxTri>& tris) const {
+        nodes.resize(nodes_.size());
+        for (size_t i = 0; i < nodes_.size(); ++i) {
+            const Node& a = nodes_[i];
+            VoxNode& b = nodes[i];
+            for (int k = 0; k < 3; ++k) { b.lo[k] = a.lo.v[k]; b.hi[k] = a.hi.v[k]; }
+            b.left = a.left; b.right = a.right; b.first = a.first; b.count = a.count;
+        }
+        tris.resize(tris_.size());
+        for (size_t i = 0; i < tris_.size(); ++i)
+            for (int k = 0; k < 3; ++k) {
+                tris[i].a[k] = tris_[i].a.v[k]; tris[i].b[k] = tris_[i].b.v[k]; tris[i].c[k] = tris_[i].c.v[k];
+            }
+    }
+
     // parity of the number of crossings of the ray o + t d, t > 1e-7
     int crossings(const Vec& o, const Vec& d) const {
         const float inf = std::numeric_limits<float>::infinity();
@@ -132,8 +147,8 @@ private:
 
 }  // namespace
 
-int voxelize_host(const float* v, int64_t ntris, int nx, int ny, int nz, const float mn[3], const float cs[3],
-                  int32_t* out) {
+namespace {
+std::vector<Tri> usable_triangles(const float* v, int64_t ntris) {
     std::vector<Tri> tris;
     tris.reserve((size_t)std::max<int64_t>(ntris, 0));
     for (int64_t t = 0; t < ntris; ++t) {
@@ -156,13 +171,34 @@ int voxelize_host(const float* v, int64_t ntris, int nx, int ny, int nz, const f
         if (!(len > 1e-14f) || !std::isfinite(len)) continue;
         tris.push_back(tr);
     }
-    const TriBvh bvh(std::move(tris));
+    return tris;
+}
+}  // namespace
+
+void voxel_jitter(const float cs[3], float jit[3]) {
+    jit[0] = 0.731f * 1e-3f * cs[0];
+    jit[1] = 0.377f * 1e-3f * cs[1];
+    jit[2] = 0.557f * 1e-3f * cs[2];
+}
+
+void voxel_bvh(const float* v, int64_t ntris, std::vector<VoxNode>& nodes, std::vector<VoxTri>& tris) {
+    const TriBvh bvh(usable_triangles(v, ntris));
+    nodes.clear();
+    tris.clear();
+    if (!bvh.empty()) bvh.flatten(nodes, tris);
+}
+
+int voxelize_host(const float* v, int64_t ntris, int nx, int ny, int nz, const float mn[3], const float cs[3],
+                  int32_t* out) {
+    const TriBvh bvh(usable_triangles(v, ntris));
     const int64_t total = (int64_t)nx * ny * nz;
     if (bvh.empty()) {
         std::fill(out, out + total, 0);
         return AEROS_OK;
     }
-    const float jx = 0.731f * 1e-3f * cs[0], jy = 0.377f * 1e-3f * cs[1], jz = 0.557f * 1e-3f * cs[2];
+    float jit[3];
+    voxel_jitter(cs, jit);
+    const float jx = jit[0], jy = jit[1], jz = jit[2];
     const Vec dx{{1, 0, 0}}, dy{{0, 1, 0}}, dz{{0, 0, 1}};
     auto work = [&](int64_t rowStart, int64_t rowEnd) {
         for (int64_t r = rowStart; r < rowEnd; ++r) {   // r = k*ny + j

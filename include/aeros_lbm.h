@@ -284,6 +284,16 @@ int aeros_lbm_equilibrium_moments(float rho, float ux, float uy, float uz,
 int aeros_voxelize(const float* tris, int64_t ntris, int32_t nx, int32_t ny, int32_t nz,
                    const float box_min[3], const float cell[3], int32_t* out);
 
+/* The same vote on the device (replaces the per-voxel ray loop of
+ * buildVoxelGrid, aeros/src/voxel_grid.cpp:337-351; the BVH of
+ * aeros/src/mesh_bvh.h:101-160 is built on the host and uploaded): one
+ * thread per voxel, identical float slab and double-precision triangle
+ * tests, so out is bit-identical to aeros_voxelize.  Synchronous; borrows
+ * tris and out for the call only.  AEROS_ERR_NO_DEVICE without a gfx950
+ * device (no host fallback). */
+int aeros_voxelize_gpu(const float* tris, int64_t ntris, int32_t nx, int32_t ny, int32_t nz,
+                       const float box_min[3], const float cell[3], int32_t device, int32_t* out);
+
 /* buildVoxelGrid's grid plan (aeros/src/voxel_grid.cpp:225-300): bbox plus
  * 0.14*maxDim margin, nx = res, ny/nz proportional.  Fills dims and box. */
 int aeros_voxel_plan(const float bb_min[3], const float bb_max[3], float max_dim, int32_t res,
