@@ -216,6 +216,17 @@ class ServeStats(C.Structure):
 REQ_STATES = {0: "queued", 1: "running", 2: "done", 3: "limit", 4: "failed"}
 
 
+class StlInfo(C.Structure):
+    """aeros_stl_info: stlformat::Detection (aeros/src/stl_format.h:36-42)."""
+    _fields_ = [("format", C.c_int32), ("reason", C.c_int32), ("declared_triangles", C.c_uint32),
+                ("pad", C.c_int32), ("payload_bytes", C.c_int64), ("trailing_bytes", C.c_int64)]
+
+
+STL_FORMATS = {0: "unknown", 1: "binary", 2: "ascii"}
+STL_REASONS = {1: "empty", 2: "solid+keywords", 3: "binary exact", 4: "binary with tail", 5: "text without solid",
+               6: "text without keywords", 7: "truncated", 8: "unrecognised"}
+
+
 def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
     """Load libaeros_lbm.so and declare its prototypes.  Raises if it is missing."""
     global _lib
@@ -252,6 +263,12 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
                                      P(C.c_float), P(C.c_float), P(C.c_int32)]),
         "aeros_voxelize_gpu": (C.c_int, [P(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                          P(C.c_float), P(C.c_float), C.c_int32, P(C.c_int32)]),
+        "aeros_stl_detect": (C.c_int, [C.c_char_p, C.c_int64, P(StlInfo)]),
+        "aeros_stl_parse": (C.c_int, [C.c_char_p, C.c_int64, P(P(C.c_float)), P(P(C.c_float)), P(C.c_int64),
+                                      P(C.c_int64)]),
+        "aeros_stl_load": (C.c_int, [C.c_char_p, P(P(C.c_float)), P(P(C.c_float)), P(C.c_int64), P(C.c_int64)]),
+        "aeros_stl_free": (None, [P(C.c_float)]),
+        "aeros_stl_write_binary": (C.c_int, [C.c_char_p, P(C.c_float), C.c_int64, C.c_char_p]),
         "aeros_voxel_plan": (C.c_int, [P(C.c_float), P(C.c_float), C.c_float, C.c_int32,
                                        P(C.c_int32), P(C.c_float), P(C.c_float)]),
         "aeros_lbm_last_error": (C.c_int, [C.c_char_p, C.c_int32]),
@@ -462,6 +479,53 @@ def voxelize_gpu(tris, dims, box_min, cell, device: int = 0):
     _check(lib.aeros_voxelize_gpu(tris.ctypes.data_as(C.POINTER(C.c_float)), tris.shape[0], nx, ny, nz, mn, cs,
                                   int(device), out.ctypes.data_as(C.POINTER(C.c_int32))), "aeros_voxelize_gpu")
     return out.reshape(nz, ny, nx)
+
+
+def stl_detect(data: bytes) -> dict:
+    """Format of an STL buffer (stlformat::detect)."""
+    lib = load_library()
+    info = StlInfo()
+    _check(lib.aeros_stl_detect(data, len(data), C.byref(info)), "aeros_stl_detect")
+    return dict(format=STL_FORMATS[info.format], reason=STL_REASONS.get(info.reason, str(info.reason)),
+                declared_triangles=info.declared_triangles, payload_bytes=info.payload_bytes,
+                trailing_bytes=info.trailing_bytes)
+
+
+def _stl_take(lib, rc, what, pt, pn, nt, dropped):
+    _check(rc, what)
+    try:
+        n = nt.value
+        tris = np.ctypeslib.as_array(pt, shape=(n * 9,)).reshape(n, 9).copy()
+        norms = np.ctypeslib.as_array(pn, shape=(n * 9,)).reshape(n, 9).copy()
+    finally:
+        lib.aeros_stl_free(pt)
+        lib.aeros_stl_free(pn)
+    return tris, norms, dropped.value
+
+
+def stl_parse(data: bytes):
+    """loadSTL on a buffer: ((T, 9) vertices, (T, 9) per-vertex normals, dropped count)."""
+    lib = load_library()
+    pt, pn = C.POINTER(C.c_float)(), C.POINTER(C.c_float)()
+    nt, dropped = C.c_int64(), C.c_int64()
+    rc = lib.aeros_stl_parse(data, len(data), C.byref(pt), C.byref(pn), C.byref(nt), C.byref(dropped))
+    return _stl_take(lib, rc, "aeros_stl_parse", pt, pn, nt, dropped)
+
+
+def stl_load(path):
+    """loadSTL from a file."""
+    lib = load_library()
+    pt, pn = C.POINTER(C.c_float)(), C.POINTER(C.c_float)()
+    nt, dropped = C.c_int64(), C.c_int64()
+    rc = lib.aeros_stl_load(os.fsencode(path), C.byref(pt), C.byref(pn), C.byref(nt), C.byref(dropped))
+    return _stl_take(lib, rc, "aeros_stl_load", pt, pn, nt, dropped)
+
+
+def stl_write_binary(path, tris, header: str = ""):
+    lib = load_library()
+    tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+    _check(lib.aeros_stl_write_binary(os.fsencode(path), tris.ctypes.data_as(C.POINTER(C.c_float)), tris.shape[0],
+                                      header.encode()), "aeros_stl_write_binary")
 
 
 class Server:

@@ -970,6 +970,95 @@ int aeros_voxelize_gpu(const float* tris, int64_t ntris, int32_t nx, int32_t ny,
     return AEROS_OK;
 }
 
+int aeros_stl_detect(const uint8_t* data, int64_t size, aeros_stl_info* out) {
+    if (!out || size < 0 || (!data && size > 0)) return set_err(AEROS_ERR_ARG, "bad STL detect arguments");
+    stl_detect(data, (size_t)size, out);
+    return AEROS_OK;
+}
+
+namespace {
+int stl_hand_over(const std::vector<float>& v, const std::vector<float>& nv, float** tris, float** normals,
+                  int64_t* ntris) {
+    float* a = (float*)std::malloc(std::max<size_t>(v.size(), 1) * sizeof(float));
+    float* b = (float*)std::malloc(std::max<size_t>(nv.size(), 1) * sizeof(float));
+    if (!a || !b) {
+        std::free(a);
+        std::free(b);
+        return set_err(AEROS_ERR_NOMEM, "STL output allocation failed");
+    }
+    std::memcpy(a, v.data(), v.size() * sizeof(float));
+    std::memcpy(b, nv.data(), nv.size() * sizeof(float));
+    *tris = a;
+    *normals = b;
+    *ntris = (int64_t)(v.size() / 9);
+    return AEROS_OK;
+}
+}  // namespace
+
+int aeros_stl_parse(const uint8_t* data, int64_t size, float** tris, float** normals, int64_t* ntris,
+                    int64_t* dropped) {
+    if (!tris || !normals || !ntris || size < 0 || (!data && size > 0)) return set_err(AEROS_ERR_ARG, "bad STL arguments");
+    *tris = *normals = nullptr;
+    *ntris = 0;
+    std::vector<float> v, nv;
+    int64_t bad = 0;
+    const int rc = stl_parse(data, (size_t)size, v, nv, &bad);
+    if (dropped) *dropped = bad;
+    if (rc) {
+        aeros_stl_info info;
+        stl_detect(data, (size_t)size, &info);
+        return set_err(rc, "STL not loaded: format %d (reason %d), %lld triangles dropped", info.format, info.reason,
+                       (long long)bad);
+    }
+    return stl_hand_over(v, nv, tris, normals, ntris);
+}
+
+int aeros_stl_load(const char* path, float** tris, float** normals, int64_t* ntris, int64_t* dropped) {
+    if (!path || !tris || !normals || !ntris) return set_err(AEROS_ERR_ARG, "bad STL arguments");
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return set_err(AEROS_ERR_ARG, "cannot open %s", path);
+    std::fseek(fp, 0, SEEK_END);
+    const long sz = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    if (sz < 10 || sz > (2L << 30)) {
+        std::fclose(fp);
+        return set_err(AEROS_ERR_ARG, "STL file size %ld outside 10 B .. 2 GiB", sz);
+    }
+    std::vector<uint8_t> buf((size_t)sz);
+    const size_t got = std::fread(buf.data(), 1, buf.size(), fp);
+    std::fclose(fp);
+    if (got != buf.size()) return set_err(AEROS_ERR_ARG, "short read of %s (%zu of %ld bytes)", path, got, sz);
+    return aeros_stl_parse(buf.data(), (int64_t)buf.size(), tris, normals, ntris, dropped);
+}
+
+void aeros_stl_free(float* p) { std::free(p); }
+
+int aeros_stl_write_binary(const char* path, const float* tris, int64_t ntris, const char* header) {
+    if (!path || ntris < 0 || ntris > 0xffffffffLL || (!tris && ntris > 0)) return set_err(AEROS_ERR_ARG, "bad STL writer arguments");
+    std::vector<uint8_t> buf(84 + (size_t)ntris * 50, 0);
+    if (header) std::memcpy(buf.data(), header, std::min<size_t>(std::strlen(header), 80));
+    const uint32_t nt = (uint32_t)ntris;
+    for (int k = 0; k < 4; ++k) buf[80 + k] = (uint8_t)(nt >> (8 * k));
+    for (int64_t t = 0; t < ntris; ++t) {
+        const float* a = tris + t * 9;
+        const float e1[3] = {a[3] - a[0], a[4] - a[1], a[5] - a[2]};
+        const float e2[3] = {a[6] - a[0], a[7] - a[1], a[8] - a[2]};
+        float n[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2], e1[0] * e2[1] - e1[1] * e2[0]};
+        const float len = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        if (len > 0.0f && std::isfinite(len)) { n[0] /= len; n[1] /= len; n[2] /= len; }
+        else { n[0] = 0; n[1] = 0; n[2] = 1; }
+        uint8_t* p = buf.data() + 84 + (size_t)t * 50;
+        std::memcpy(p, n, 12);
+        std::memcpy(p + 12, a, 36);
+    }
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return set_err(AEROS_ERR_ARG, "cannot create %s", path);
+    const size_t put = std::fwrite(buf.data(), 1, buf.size(), fp);
+    const int closed = std::fclose(fp);
+    if (put != buf.size() || closed != 0) return set_err(AEROS_ERR_ARG, "short write of %s", path);
+    return AEROS_OK;
+}
+
 // buildVoxelGrid's grid (aeros/src/voxel_grid.cpp:250-300, voxelPlan at 172-223)
 int aeros_voxel_plan(const float bb_min[3], const float bb_max[3], float max_dim, int32_t res, int32_t dims[3],
                      float box_min[3], float box_max[3]) {

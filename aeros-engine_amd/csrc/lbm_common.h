@@ -2,6 +2,7 @@
 // the parameter blocks handed to the HIP kernels by value.
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 #include <vector>
 
@@ -88,6 +89,12 @@ struct VoxTri {
 void voxel_bvh(const float* tris, int64_t ntris, std::vector<VoxNode>& nodes, std::vector<VoxTri>& out_tris);
 // Ray jitter of the three-axis vote for a grid of cell size cs.
 void voxel_jitter(const float cs[3], float jit[3]);
+
+// STL (host): format of a buffer, and its triangles (9 floats each) with
+// per-vertex normals; returns AEROS_OK or AEROS_ERR_ARG.
+void stl_detect(const unsigned char* d, size_t size, aeros_stl_info* out);
+int stl_parse(const unsigned char* d, size_t size, std::vector<float>& verts, std::vector<float>& normals,
+              int64_t* dropped);
 
 // ----------------------------------------------------------------- kernels
 template <class R> struct StepParams {

@@ -294,6 +294,53 @@ int aeros_voxelize(const float* tris, int64_t ntris, int32_t nx, int32_t ny, int
 int aeros_voxelize_gpu(const float* tris, int64_t ntris, int32_t nx, int32_t ny, int32_t nz,
                        const float box_min[3], const float cell[3], int32_t device, int32_t* out);
 
+/* ---------------------------------------------------------------- STL */
+
+/* Format of an STL buffer, restating stlformat::detect
+ * (aeros/src/stl_format.h:86-144): ASCII when it starts with "solid" and its
+ * text holds facet/vertex; binary when the count at byte 80 promises
+ * triangles the buffer holds (a tail is allowed and counted). */
+#define AEROS_STL_UNKNOWN 0
+#define AEROS_STL_BINARY  1
+#define AEROS_STL_ASCII   2
+/* reasons */
+#define AEROS_STL_EMPTY            1
+#define AEROS_STL_SOLID_KEYWORDS   2   /* "solid" and facet/vertex: ASCII           */
+#define AEROS_STL_BINARY_EXACT     3
+#define AEROS_STL_BINARY_TAIL      4   /* binary with bytes after the triangles     */
+#define AEROS_STL_TEXT_NO_SOLID    5   /* text with keywords but no "solid": ASCII  */
+#define AEROS_STL_TEXT_NO_KEYWORDS 6   /* text without facet/vertex: unknown        */
+#define AEROS_STL_TRUNCATED        7   /* header promises more triangles than held  */
+#define AEROS_STL_UNRECOGNISED     8
+
+typedef struct aeros_stl_info {
+    int32_t  format;
+    int32_t  reason;
+    uint32_t declared_triangles;   /* count at byte 80 (binary)            */
+    int32_t  pad;
+    int64_t  payload_bytes;        /* 50 per declared triangle (binary)    */
+    int64_t  trailing_bytes;       /* bytes after the last triangle        */
+} aeros_stl_info;
+
+int aeros_stl_detect(const uint8_t* data, int64_t size, aeros_stl_info* out);
+
+/* loadSTL (aeros/src/stl_loader.cpp:30-205) on a buffer: *tris and *normals
+ * receive library-allocated arrays of 9 floats per triangle (vertex normals,
+ * the facet normal repeated; free both with aeros_stl_free).  Binary: facet
+ * normals sanitised and normalised, triangles with a non-finite coordinate
+ * dropped.  ASCII: vertices read in threes, non-finite ones dropped, normals
+ * recomputed from the edges.  *dropped counts what was dropped.  Returns
+ * AEROS_ERR_ARG when the format is unknown or no triangle remains. */
+int aeros_stl_parse(const uint8_t* data, int64_t size, float** tris, float** normals, int64_t* ntris,
+                    int64_t* dropped);
+/* The same from a file (10 B .. 2 GiB, as loadSTL accepts). */
+int aeros_stl_load(const char* path, float** tris, float** normals, int64_t* ntris, int64_t* dropped);
+void aeros_stl_free(float* p);
+/* Binary STL writer (no counterpart in the reference, which only reads):
+ * 80-byte header (header text, zero padded), count, 50 bytes per triangle
+ * with the unit normal of its edges ((0,0,1) when degenerate). */
+int aeros_stl_write_binary(const char* path, const float* tris, int64_t ntris, const char* header);
+
 /* buildVoxelGrid's grid plan (aeros/src/voxel_grid.cpp:225-300): bbox plus
  * 0.14*maxDim margin, nx = res, ny/nz proportional.  Fills dims and box. */
 int aeros_voxel_plan(const float bb_min[3], const float bb_max[3], float max_dim, int32_t res,
