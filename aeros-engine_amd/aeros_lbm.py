@@ -27,12 +27,13 @@ PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp16": FP16, "bf16": BF16}
 
 (FIELD_RHO, FIELD_UX, FIELD_UY, FIELD_UZ, FIELD_UX_WORLD, FIELD_UY_WORLD, FIELD_UZ_WORLD,
  FIELD_PRESSURE, FIELD_TKE, FIELD_STRAIN, FIELD_SOLID, FIELD_GROUND, FIELD_WALLDIST,
- FIELD_T, FIELD_BUOYANCY) = range(15)
+ FIELD_T, FIELD_BUOYANCY, FIELD_VORTICITY, FIELD_QCRIT) = range(17)
 FIELD_NAMES = {
     "rho": FIELD_RHO, "ux": FIELD_UX, "uy": FIELD_UY, "uz": FIELD_UZ,
     "ux_world": FIELD_UX_WORLD, "uy_world": FIELD_UY_WORLD, "uz_world": FIELD_UZ_WORLD,
     "pressure": FIELD_PRESSURE, "tke": FIELD_TKE, "strain": FIELD_STRAIN, "solid": FIELD_SOLID,
     "ground": FIELD_GROUND, "walldist": FIELD_WALLDIST, "T": FIELD_T, "buoyancy": FIELD_BUOYANCY,
+    "vorticity": FIELD_VORTICITY, "qcrit": FIELD_QCRIT,
 }
 
 ERRORS = {0: "ok", -1: "bad argument", -2: "out of memory", -3: "HIP error", -4: "bad state",
@@ -264,6 +265,9 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
         "aeros_voxelize_gpu": (C.c_int, [P(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                          P(C.c_float), P(C.c_float), C.c_int32, P(C.c_int32)]),
         "aeros_stl_detect": (C.c_int, [C.c_char_p, C.c_int64, P(StlInfo)]),
+        "aeros_lbm_vorticity_q": (C.c_int, [vp]),
+        "aeros_lbm_sample": (C.c_int, [vp, C.c_int32, P(C.c_float), C.c_int64, P(C.c_float), P(C.c_uint8)]),
+        "aeros_lbm_sample_velocity": (C.c_int, [vp, C.c_int32, P(C.c_float), C.c_int64, P(C.c_float)]),
         "aeros_stl_parse": (C.c_int, [C.c_char_p, C.c_int64, P(P(C.c_float)), P(P(C.c_float)), P(C.c_int64),
                                       P(C.c_int64)]),
         "aeros_stl_load": (C.c_int, [C.c_char_p, P(P(C.c_float)), P(P(C.c_float)), P(C.c_int64), P(C.c_int64)]),
@@ -439,6 +443,30 @@ class Lattice:
 
     def thermal_reset(self):
         _check(self.lib.aeros_lbm_thermal_reset(self.h), "aeros_lbm_thermal_reset")
+
+    def vorticity_q(self):
+        """computeLBMVorticityAndQ on the device (fields "vorticity", "qcrit", "strain")."""
+        _check(self.lib.aeros_lbm_vorticity_q(self.h), "aeros_lbm_vorticity_q")
+
+    def sample(self, name: str, pos):
+        """World-space trilinear sampler of a scalar field at (N, 3) points: (values, valid)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        out = np.empty(pos.shape[0], dtype=np.float32)
+        valid = np.empty(pos.shape[0], dtype=np.uint8)
+        F = C.POINTER(C.c_float)
+        _check(self.lib.aeros_lbm_sample(self.h, FIELD_NAMES[name], pos.ctypes.data_as(F), pos.shape[0],
+                                         out.ctypes.data_as(F), valid.ctypes.data_as(C.POINTER(C.c_uint8))),
+               "aeros_lbm_sample")
+        return out, valid.astype(bool)
+
+    def sample_velocity(self, pos, world: bool = True):
+        """getLBMVelocityWorld (world) or getLBMVelocityLB at (N, 3) points: (N, 3)."""
+        pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
+        out = np.empty((pos.shape[0], 3), dtype=np.float32)
+        F = C.POINTER(C.c_float)
+        _check(self.lib.aeros_lbm_sample_velocity(self.h, 1 if world else 0, pos.ctypes.data_as(F), pos.shape[0],
+                                                  out.ctypes.data_as(F)), "aeros_lbm_sample_velocity")
+        return out
 
     def kernel_timing(self, enable: int):
         """enable 1/0/-1 (start/stop/reset); returns (total ms, launches) of the step kernel."""

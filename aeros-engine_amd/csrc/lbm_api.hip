@@ -103,6 +103,9 @@ struct EngineBase {
     virtual int thermal_upload(const float* T) = 0;
     virtual int projected_area(int axis, float* out) = 0;
     virtual void scalars(aeros_lbm_scalars* o) = 0;
+    virtual int vorticity_q() = 0;
+    virtual int sample(int field, const float* pos, int64_t np, float* out, uint8_t* valid) = 0;
+    virtual int sample_velocity(int world, const float* pos, int64_t np, float* out) = 0;
     // HIP-event timing of the step kernel launches
     bool timing = false;
     double timed_ms = 0.0;
@@ -128,6 +131,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
     int64_t n = 0;
     int grid = 1;
     DevBuf f[2], mask, solid_u8, u[2], fld[6], world[4], g[2], T[2], buoy[2], parts, acc, status, prev_rho, scratch;
+    DevBuf vort, qcrit;   // computeLBMVorticityAndQ outputs, allocated on first use
     bool pre = true;          // buffers hold pre-collision populations (after init / reset)
     int cur = 0;              // lbmCurrentStep (host mirror)
     int resets = 0;
@@ -563,6 +567,16 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             case AEROS_FIELD_UY_WORLD: src = &world[1]; break;
             case AEROS_FIELD_UZ_WORLD: src = &world[2]; break;
             case AEROS_FIELD_PRESSURE: src = &world[3]; break;
+            case AEROS_FIELD_VORTICITY:
+            case AEROS_FIELD_QCRIT: {
+                const DevBuf& b = id == AEROS_FIELD_VORTICITY ? vort : qcrit;
+                if (!b.p) {   // never computed: zeros, as the reference's freshly allocated arrays
+                    for (int64_t i = 0; i < n; ++i) { if (d64) d64[i] = 0.0; if (d32) d32[i] = 0.0f; }
+                    return AEROS_OK;
+                }
+                src = &b;
+                break;
+            }
             case AEROS_FIELD_T:
             case AEROS_FIELD_BUOYANCY:
                 if (!s.thermal_ready) return set_err(AEROS_ERR_STATE, "thermal module is off");
@@ -608,6 +622,91 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         HIP_TRY(hipStreamSynchronize(st));
         for (int64_t c = 0; c < n; ++c)
             for (int q = 0; q < Q; ++q) out[c * Q + q] = (double)h_[(size_t)q * n + c];
+        return AEROS_OK;
+    }
+
+    // computeLBMVorticityAndQ (aeros/src/lbm.cpp:2005-2093) on the device
+    int vorticity_q() override {
+        int rc;
+        if (!vort.p) {
+            if ((rc = vort.alloc((size_t)n * sizeof(R))) || (rc = qcrit.alloc((size_t)n * sizeof(R)))) return rc;
+            if ((rc = fill<R>(vort, n, R(0))) || (rc = fill<R>(qcrit, n, R(0)))) return rc;
+        }
+        float cs[3];
+        for (int k = 0; k < 3; ++k) cs[k] = s.cell[k] < 1e-6f ? 0.1f : s.cell[k];
+        hipLaunchKernelGGL((k_vort_q<R>), dim3(grid_for(n)), dim3(256), 0, st, mask.as<uint32_t>(), world[0].as<R>(),
+                           world[1].as<R>(), world[2].as<R>(), vort.as<R>(), qcrit.as<R>(), fld[5].as<R>(), s.nx, s.ny,
+                           s.nz, 1.0f / cs[0], 1.0f / cs[1], 1.0f / cs[2]);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(st));
+        return AEROS_OK;
+    }
+
+    SampleGrid sample_grid() const {
+        SampleGrid G;
+        G.minx = s.min[0]; G.miny = s.min[1]; G.minz = s.min[2];
+        G.csx = s.cell[0]; G.csy = s.cell[1]; G.csz = s.cell[2];
+        G.maxx = s.min[0] + (float)s.nx * s.cell[0];
+        G.maxy = s.min[1] + (float)s.ny * s.cell[1];
+        G.maxz = s.min[2] + (float)s.nz * s.cell[2];
+        G.nx = s.nx; G.ny = s.ny; G.nz = s.nz;
+        return G;
+    }
+
+    // Batched world-space samplers (aeros/src/lbm.cpp:1845-1999) on the device.
+    int sample(int field, const float* pos, int64_t np, float* out, uint8_t* valid) override {
+        const R* src = nullptr;
+        int mode = 0;
+        switch (field) {
+            case AEROS_FIELD_PRESSURE: src = world[3].as<R>(); break;
+            case AEROS_FIELD_RHO: src = fld[0].as<R>(); mode = 1; break;
+            case AEROS_FIELD_TKE: src = fld[4].as<R>(); break;
+            case AEROS_FIELD_STRAIN: src = fld[5].as<R>(); break;
+            case AEROS_FIELD_VORTICITY:
+            case AEROS_FIELD_QCRIT: {
+                int rc;
+                if (!vort.p && (rc = vorticity_q())) return rc;
+                src = (field == AEROS_FIELD_VORTICITY ? vort : qcrit).as<R>();
+                break;
+            }
+            default: return set_err(AEROS_ERR_ARG, "field %d has no world sampler", field);
+        }
+        if (np <= 0) return AEROS_OK;
+        DevBuf dp, dout, dval;
+        int rc;
+        if ((rc = dp.alloc((size_t)np * 3 * sizeof(float))) || (rc = dout.alloc((size_t)np * sizeof(float))) ||
+            (rc = dval.alloc((size_t)np)))
+            return rc;
+        HIP_TRY(hipMemcpyAsync(dp.p, pos, (size_t)np * 3 * sizeof(float), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL((k_sample<R>), dim3(grid_for(np)), dim3(256), 0, st, src, mask.as<uint32_t>(), sample_grid(),
+                           dp.as<float>(), np, mode, dout.as<float>(), dval.as<uint8_t>());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)np * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (valid) HIP_TRY(hipMemcpyAsync(valid, dval.p, (size_t)np, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return AEROS_OK;
+    }
+
+    int sample_velocity(int world_units, const float* pos, int64_t np, float* out) override {
+        if (np <= 0) return AEROS_OK;
+        float in[3];
+        inlet_velocity(s.cfg, in[0], in[1], in[2]);
+        float U0 = std::sqrt(in[0] * in[0] + in[1] * in[1] + in[2] * in[2]);
+        if (!std::isfinite(U0) || U0 < 1e-6f) U0 = 0.08f;
+        float scale = s.cfg.flow_speed / U0;
+        if (!std::isfinite(scale) || scale > 1000.0f) scale = 20.0f;
+        float fs[3];
+        free_stream(s.cfg, fs);
+        DevBuf dp, dout;
+        int rc;
+        if ((rc = dp.alloc((size_t)np * 3 * sizeof(float))) || (rc = dout.alloc((size_t)np * 3 * sizeof(float)))) return rc;
+        HIP_TRY(hipMemcpyAsync(dp.p, pos, (size_t)np * 3 * sizeof(float), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL((k_sample_velocity<R>), dim3(grid_for(np)), dim3(256), 0, st, fld[1].as<R>(), fld[2].as<R>(),
+                           fld[3].as<R>(), mask.as<uint32_t>(), sample_grid(), dp.as<float>(), np, world_units ? 1 : 0,
+                           scale, s.cfg.flow_speed * 2.0f, fs[0], fs[1], fs[2], dout.as<float>());
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)np * 3 * sizeof(float), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
         return AEROS_OK;
     }
 
@@ -968,6 +1067,24 @@ int aeros_voxelize_gpu(const float* tris, int64_t ntris, int32_t nx, int32_t ny,
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpy(out, dout.p, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
     return AEROS_OK;
+}
+
+int aeros_lbm_vorticity_q(aeros_lbm* h) {
+    if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
+    HIP_TRY(hipSetDevice(h->device));
+    return h->eng->vorticity_q();
+}
+
+int aeros_lbm_sample(aeros_lbm* h, int32_t field, const float* pos, int64_t npos, float* out, uint8_t* valid) {
+    if (!h || !h->eng || npos < 0 || (npos > 0 && (!pos || !out))) return set_err(AEROS_ERR_ARG, "bad sample arguments");
+    HIP_TRY(hipSetDevice(h->device));
+    return h->eng->sample(field, pos, npos, out, valid);
+}
+
+int aeros_lbm_sample_velocity(aeros_lbm* h, int32_t world, const float* pos, int64_t npos, float* out) {
+    if (!h || !h->eng || npos < 0 || (npos > 0 && (!pos || !out))) return set_err(AEROS_ERR_ARG, "bad sample arguments");
+    HIP_TRY(hipSetDevice(h->device));
+    return h->eng->sample_velocity(world, pos, npos, out);
 }
 
 int aeros_stl_detect(const uint8_t* data, int64_t size, aeros_stl_info* out) {

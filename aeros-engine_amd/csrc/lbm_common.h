@@ -65,6 +65,8 @@ int setup_after_mask(Setup& s, char* msg, int msglen);
 void config_defaults(aeros_lbm_config* c);
 void inlet_velocity(const aeros_lbm_config& c, float& ux, float& uy, float& uz);
 void flow_axis(const aeros_lbm_config& c, int& axis, int& sign);
+// The free stream in world units (flowParams.vx/vy/vz).
+void free_stream(const aeros_lbm_config& c, float v[3]);
 // Initial temperature field of thermalResetField (host, glibc cos/sin).
 void thermal_initial_field(const Setup& s, std::vector<float>& T);
 // Initial per-cell velocity with the curl-noise inflow (host, glibc cos).

@@ -1345,6 +1345,185 @@ __global__ void k_thermal_diag(const R* T, const uint32_t* mask, int nx, int ny,
 
 __global__ void k_probe(int* d) { d[threadIdx.x] += 1; }
 
+// ------------------------------------------------------------ post-processing on the device
+// computeLBMVorticityAndQ (aeros/src/lbm.cpp:2005-2093): central differences
+// of the world velocity over the interior cells, in float as the reference
+// computes them; |omega| (capped at 1000), Q = (|Omega|^2 - |S|^2)/2 (capped
+// at +-1e6) and sqrt(2 S:S), which overwrites the strain field as in the
+// reference.  Solid and ground cells get 0 vorticity and Q; the shell keeps
+// its values.
+template <class R>
+__global__ void k_vort_q(const uint32_t* mask, const R* uxw, const R* uyw, const R* uzw, R* vort, R* qc, R* strain,
+                         int nx, int ny, int nz, float icx, float icy, float icz) {
+    const int64_t n = (int64_t)nx * ny * nz, sy = nx, sz = (int64_t)nx * ny;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
+        const int x = (int)(c % nx), y = (int)((c / nx) % ny), z = (int)(c / sz);
+        if (x < 1 || x > nx - 2 || y < 1 || y > ny - 2 || z < 1 || z > nz - 2) continue;
+        if (mask[c] & M_SOLID) { vort[c] = R(0); qc[c] = R(0); continue; }
+        auto d = [&](const R* f, int64_t o, float ic) { return ((float)f[c + o] - (float)f[c - o]) * 0.5f * ic; };
+        const float dux_dx = d(uxw, 1, icx), dux_dy = d(uxw, sy, icy), dux_dz = d(uxw, sz, icz);
+        const float duy_dx = d(uyw, 1, icx), duy_dy = d(uyw, sy, icy), duy_dz = d(uyw, sz, icz);
+        const float duz_dx = d(uzw, 1, icx), duz_dy = d(uzw, sy, icy), duz_dz = d(uzw, sz, icz);
+        if (!__builtin_isfinite(dux_dx)) { vort[c] = R(0); qc[c] = R(0); continue; }
+        const float wx = duz_dy - duy_dz, wy = dux_dz - duz_dx, wz = duy_dx - dux_dy;
+        float vm = __builtin_sqrtf(wx * wx + wy * wy + wz * wz);
+        if (!__builtin_isfinite(vm)) vm = 0.0f;
+        if (vm > 1000.0f) vm = 1000.0f;
+        vort[c] = (R)vm;
+        const float sxy = 0.5f * (dux_dy + duy_dx), sxz = 0.5f * (dux_dz + duz_dx), syz = 0.5f * (duy_dz + duz_dy);
+        const float oxy = 0.5f * (dux_dy - duy_dx), oxz = 0.5f * (dux_dz - duz_dx), oyz = 0.5f * (duy_dz - duz_dy);
+        const float s2 = dux_dx * dux_dx + duy_dy * duy_dy + duz_dz * duz_dz + 2.0f * (sxy * sxy + sxz * sxz + syz * syz);
+        const float o2 = 2.0f * (oxy * oxy + oxz * oxz + oyz * oyz);
+        float qv = 0.5f * (o2 - s2);
+        if (!__builtin_isfinite(qv)) qv = 0.0f;
+        if (qv > 1e6f) qv = 1e6f;
+        if (qv < -1e6f) qv = -1e6f;
+        qc[c] = (R)qv;
+        float sm = __builtin_sqrtf(2.0f * s2);
+        if (!__builtin_isfinite(sm)) sm = 0.0f;
+        strain[c] = (R)sm;
+    }
+}
+
+// The lattice as the world-space samplers see it (lbmMin*, lbmCellSize*, lbmN*).
+struct SampleGrid {
+    float minx, miny, minz, maxx, maxy, maxz, csx, csy, csz;
+    int nx, ny, nz;
+};
+__device__ inline float clamp01(float t) { return t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t); }
+
+// sampleFieldTrilinear with solid cells skipped (aeros/src/lbm.cpp:1860-1914).
+template <class R>
+__device__ bool sample_field(const R* field, const uint32_t* mask, const SampleGrid& G, float px, float py, float pz,
+                             float* out) {
+    *out = 0.0f;
+    if (!__builtin_isfinite(px) || !__builtin_isfinite(py) || !__builtin_isfinite(pz)) return false;
+    if (G.csx < 1e-9f || G.csy < 1e-9f || G.csz < 1e-9f) return false;
+    if (G.nx < 2 || G.ny < 2 || G.nz < 2) return false;
+    const float fx = (px - G.minx) / G.csx, fy = (py - G.miny) / G.csy, fz = (pz - G.minz) / G.csz;
+    const int ix = (int)__builtin_floorf(fx), iy = (int)__builtin_floorf(fy), iz = (int)__builtin_floorf(fz);
+    if (ix < 0 || iy < 0 || iz < 0) return false;
+    if (ix > G.nx - 2 || iy > G.ny - 2 || iz > G.nz - 2) return false;
+    const float tx = clamp01(fx - (float)ix), ty = clamp01(fy - (float)iy), tz = clamp01(fz - (float)iz);
+    const float wx[2] = {1.0f - tx, tx}, wy[2] = {1.0f - ty, ty}, wz[2] = {1.0f - tz, tz};
+    float acc = 0.0f, wsum = 0.0f;
+    for (int dz = 0; dz <= 1; ++dz)
+        for (int dy = 0; dy <= 1; ++dy)
+            for (int dx = 0; dx <= 1; ++dx) {
+                const int64_t c = ((int64_t)(iz + dz) * G.ny + (iy + dy)) * G.nx + (ix + dx);
+                if (mask[c] & M_SOLID) continue;
+                const float v = (float)field[c];
+                if (!__builtin_isfinite(v)) continue;
+                const float w = wx[dx] * wy[dy] * wz[dz];
+                acc += v * w;
+                wsum += w;
+            }
+    if (wsum < 1e-6f) return false;
+    acc /= wsum;
+    if (!__builtin_isfinite(acc)) return false;
+    *out = acc;
+    return true;
+}
+
+// Scalar samplers: mode 0 = value or 0 (getLBMPressureWorld, getLBMVorticityWorld,
+// getLBMQWorld, getLBMTKEWorld, getLBMStrainWorld), mode 1 = density clamped
+// to [0.8, 1.2] or 1 (getLBMDensityWorld, aeros/src/lbm.cpp:1942-1999).
+template <class R>
+__global__ void k_sample(const R* field, const uint32_t* mask, SampleGrid G, const float* pos, int64_t np, int mode,
+                         float* out, uint8_t* valid) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        float v = 0.0f;
+        const bool ok = sample_field(field, mask, G, pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], &v);
+        if (mode == 1) v = ok ? (v < 0.8f ? 0.8f : (v > 1.2f ? 1.2f : v)) : 1.0f;
+        else if (!ok) v = 0.0f;
+        out[i] = v;
+        if (valid) valid[i] = ok ? 1 : 0;
+    }
+}
+
+// getLBMVelocityLB (aeros/src/lbm.cpp:1781-1843): trilinear over the fluid
+// corners only, weights renormalised.
+template <class R>
+__device__ void velocity_lb(const R* ux, const R* uy, const R* uz, const uint32_t* mask, const SampleGrid& G, float px,
+                            float py, float pz, float v[3]) {
+    v[0] = v[1] = v[2] = 0.0f;
+    // (the reference tests x only; a non-finite y or z floors to INT_MIN on
+    // its x86 build and returns 0 the same way)
+    if (!__builtin_isfinite(px) || !__builtin_isfinite(py) || !__builtin_isfinite(pz)) return;
+    const float fx = (px - G.minx) / G.csx, fy = (py - G.miny) / G.csy, fz = (pz - G.minz) / G.csz;
+    const int ix = (int)__builtin_floorf(fx), iy = (int)__builtin_floorf(fy), iz = (int)__builtin_floorf(fz);
+    const float tx = clamp01(fx - (float)ix), ty = clamp01(fy - (float)iy), tz = clamp01(fz - (float)iz);
+    if (ix < 0 || ix >= G.nx - 1 || iy < 0 || iy >= G.ny - 1 || iz < 0 || iz >= G.nz - 1) return;
+    const float w[8] = {(1 - tx) * (1 - ty) * (1 - tz), tx * (1 - ty) * (1 - tz), (1 - tx) * ty * (1 - tz), tx * ty * (1 - tz),
+                        (1 - tx) * (1 - ty) * tz,       tx * (1 - ty) * tz,       (1 - tx) * ty * tz,       tx * ty * tz};
+    float sv[8][3], sw[8], tot = 0.0f;
+    int cnt = 0;
+    for (int k = 0; k < 8; ++k) {   // corner order 000, 100, 010, 110, 001, 101, 011, 111
+        const int x = ix + (k & 1), y = iy + ((k >> 1) & 1), z = iz + ((k >> 2) & 1);
+        const int64_t c = ((int64_t)z * G.ny + y) * G.nx + x;
+        if (mask[c] & M_SOLID) continue;
+        const float a = (float)ux[c], b = (float)uy[c], d = (float)uz[c];
+        if (!__builtin_isfinite(a)) continue;
+        sv[cnt][0] = a; sv[cnt][1] = b; sv[cnt][2] = d;
+        sw[cnt] = w[k];
+        tot += w[k];
+        cnt++;
+    }
+    if (cnt == 0) return;
+    if (tot < 1e-6f) { v[0] = sv[0][0]; v[1] = sv[0][1]; v[2] = sv[0][2]; return; }
+    float r[3] = {0.0f, 0.0f, 0.0f};
+    for (int i = 0; i < cnt; ++i) {
+        const float k = sw[i] / tot;
+        r[0] += sv[i][0] * k; r[1] += sv[i][1] * k; r[2] += sv[i][2] * k;
+    }
+    if (__builtin_isfinite(r[0])) { v[0] = r[0]; v[1] = r[1]; v[2] = r[2]; }
+}
+
+// world = 0: getLBMVelocityLB; world = 1: getLBMVelocityWorld (aeros/src/lbm.cpp:
+// 1845-1869): the free stream outside the lattice box, 0 inside a solid cell
+// and where |u_LB|^2 < 1e-12, else u_LB flowSpeed/U0, capped at 2 flowSpeed.
+template <class R>
+__global__ void k_sample_velocity(const R* ux, const R* uy, const R* uz, const uint32_t* mask, SampleGrid G,
+                                  const float* pos, int64_t np, int world, float scale, float vmax, float fsx, float fsy,
+                                  float fsz, float* out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < np; i += (int64_t)gridDim.x * blockDim.x) {
+        const float px = pos[3 * i], py = pos[3 * i + 1], pz = pos[3 * i + 2];
+        float v[3];
+        if (!world) {
+            velocity_lb(ux, uy, uz, mask, G, px, py, pz, v);
+        } else if (px < G.minx || px > G.maxx || py < G.miny || py > G.maxy || pz < G.minz || pz > G.maxz) {
+            v[0] = fsx; v[1] = fsy; v[2] = fsz;
+        } else {
+            // isLBMSolidWorld (aeros/src/lbm.cpp:1954-1969): nearest cell by floor
+            bool solid = false;
+            if (__builtin_isfinite(px) && __builtin_isfinite(py) && __builtin_isfinite(pz) && G.csx >= 1e-9f &&
+                G.csy >= 1e-9f && G.csz >= 1e-9f) {
+                const int ix = (int)__builtin_floorf((px - G.minx) / G.csx), iy = (int)__builtin_floorf((py - G.miny) / G.csy),
+                          iz = (int)__builtin_floorf((pz - G.minz) / G.csz);
+                if (ix >= 0 && ix < G.nx && iy >= 0 && iy < G.ny && iz >= 0 && iz < G.nz)
+                    solid = (mask[((int64_t)iz * G.ny + iy) * G.nx + ix] & M_SOLID) != 0;
+            }
+            v[0] = v[1] = v[2] = 0.0f;
+            if (!solid) {
+                float l[3];
+                velocity_lb(ux, uy, uz, mask, G, px, py, pz, l);
+                const float m2 = l[0] * l[0] + l[1] * l[1] + l[2] * l[2];
+                if (!(m2 < 1e-12f)) {
+                    float w0 = l[0] * scale, w1 = l[1] * scale, w2 = l[2] * scale;
+                    if (!__builtin_isfinite(w0)) {
+                        w0 = fsx; w1 = fsy; w2 = fsz;
+                    } else {
+                        const float mag = __builtin_sqrtf(w0 * w0 + w1 * w1 + w2 * w2);
+                        if (mag > vmax) { const float k = vmax / mag; w0 *= k; w1 *= k; w2 *= k; }
+                    }
+                    v[0] = w0; v[1] = w1; v[2] = w2;
+                }
+            }
+        }
+        out[3 * i] = v[0]; out[3 * i + 1] = v[1]; out[3 * i + 2] = v[2];
+    }
+}
+
 // ------------------------------------------------------------ voxelizer
 // The three-ray majority vote of the set-mask producer (aeros/src/mesh_bvh.h:
 // 62-185, aeros/src/voxel_grid.cpp:337-351) on the device: one thread per

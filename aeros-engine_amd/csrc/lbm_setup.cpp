@@ -78,6 +78,24 @@ void inlet_velocity(const aeros_lbm_config& c, float& ux, float& uy, float& uz) 
     ux = d[0] * U; uy = d[1] * U; uz = d[2] * U;
 }
 
+// flowParams.vx/vy/vz, the free stream in world units (aeros/src/flow_field.cpp:49-68)
+void free_stream(const aeros_lbm_config& c, float v[3]) {
+    float speed = std::isfinite(c.flow_speed) ? c.flow_speed : 2.0f;
+    if (speed < 0) speed = 0;
+    if (speed > 1000.0f) speed = 1000.0f;
+    const float az0 = std::isfinite(c.flow_azimuth_deg) ? c.flow_azimuth_deg : 0.0f;
+    float el0 = std::isfinite(c.flow_elevation_deg) ? c.flow_elevation_deg : 0.0f;
+    if (el0 < -89.0f) el0 = -89.0f;
+    if (el0 > 89.0f) el0 = 89.0f;
+    const float az = deg2rad(az0), el = deg2rad(el0);
+    const float ce = cosf(el), se = sinf(el), ca = cosf(az), sa = sinf(az);
+    float d[3] = {ce * ca, se, ce * sa};
+    const float len = sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    if (len < 1e-6f || !std::isfinite(len)) { d[0] = 1; d[1] = 0; d[2] = 0; }
+    else for (float& x : d) x /= len;
+    for (int k = 0; k < 3; ++k) v[k] = d[k] * speed;
+}
+
 void flow_axis(const aeros_lbm_config& c, int& axis, int& sign) {
     const Dir r = flow_direction(c);
     const float a0 = std::fabs(r.d[0]), a1 = std::fabs(r.d[1]), a2 = std::fabs(r.d[2]);

@@ -63,7 +63,9 @@ extern "C" {
 #define AEROS_FIELD_WALLDIST  12   /* u8 wall distance (0..255) as float            */
 #define AEROS_FIELD_T         13   /* thermalT      (aeros/src/thermal_lbm.h:46)     */
 #define AEROS_FIELD_BUOYANCY  14   /* thermalBuoyancyY                              */
-#define AEROS_FIELD_COUNT     15
+#define AEROS_FIELD_VORTICITY 15   /* lbmVorticityMag, 1/s (aeros_lbm_vorticity_q)  */
+#define AEROS_FIELD_QCRIT     16   /* lbmQCriterion, 1/s^2 (aeros_lbm_vorticity_q)  */
+#define AEROS_FIELD_COUNT     17
 
 /* bit masks for aeros_lbm_read */
 #define AEROS_READ_RHO (1u << 0)
@@ -293,6 +295,27 @@ int aeros_voxelize(const float* tris, int64_t ntris, int32_t nx, int32_t ny, int
  * device (no host fallback). */
 int aeros_voxelize_gpu(const float* tris, int64_t ntris, int32_t nx, int32_t ny, int32_t nz,
                        const float box_min[3], const float cell[3], int32_t device, int32_t* out);
+
+/* ---------------------------------------------------------------- post-processing on the device */
+
+/* computeLBMVorticityAndQ (aeros/src/lbm.cpp:2005-2093): from the world
+ * velocity of the last step, |omega| and Q on the interior cells (0 in solid
+ * cells), and sqrt(2 S:S) in world units written over AEROS_FIELD_STRAIN, as
+ * the reference overwrites lbmStrainMag. */
+int aeros_lbm_vorticity_q(aeros_lbm* h);
+
+/* The world-space samplers the visualizer calls per point, batched:
+ * sampleFieldTrilinear over fluid cells (aeros/src/lbm.cpp:1871-1925) for
+ * field = AEROS_FIELD_PRESSURE (getLBMPressureWorld), RHO (getLBMDensityWorld:
+ * clamped to [0.8, 1.2], 1 when no fluid corner), TKE, STRAIN, VORTICITY,
+ * QCRIT (0 when no fluid corner).  pos holds npos (x, y, z) world points;
+ * out npos values; valid (may be NULL) 1 where the sampler found fluid data. */
+int aeros_lbm_sample(aeros_lbm* h, int32_t field, const float* pos, int64_t npos, float* out, uint8_t* valid);
+/* getLBMVelocityLB (world = 0, lattice units) or getLBMVelocityWorld
+ * (world = 1: free stream outside the lattice box, 0 in solid cells, else
+ * scaled by flowSpeed/U0 and capped at 2 flowSpeed), aeros/src/lbm.cpp:
+ * 1781-1869.  out receives 3 floats per point. */
+int aeros_lbm_sample_velocity(aeros_lbm* h, int32_t world, const float* pos, int64_t npos, float* out);
 
 /* ---------------------------------------------------------------- STL */
 

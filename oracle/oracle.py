@@ -205,3 +205,171 @@ def voxelize(tris, dims, box_min, cell):
     lib.ora_voxelize(tris.ctypes.data_as(C.POINTER(C.c_float)), tris.shape[0], nx, ny, nz, mn, cs,
                      out.ctypes.data_as(C.POINTER(C.c_int32)))
     return out.reshape(nz, ny, nx)
+
+
+# ---------------------------------------------------------------- post-processing
+# Restatements of the reference's field post-processing, in float32 with the
+# reference's operation order (numpy rounds every float32 operation), for the
+# GPU post-processing tests.
+
+f32 = np.float32
+
+
+def vorticity_q(uxw, uyw, uzw, solid, cell):
+    """computeLBMVorticityAndQ (aeros/src/lbm.cpp:2005-2093) on (nz, ny, nx) float32
+    world velocities; solid = solid | ground.  Returns (vort, q, strain, interior)
+    where vort/q are 0 on solid interior cells and strain is defined on the fluid
+    interior cells whose x-derivative of u_x is finite (the reference leaves the
+    other cells of these arrays untouched)."""
+    cs = [f32(c) if c >= 1e-6 else f32(0.1) for c in cell]
+    ic = [f32(1) / c for c in cs]
+    u = [np.asarray(a, dtype=f32) for a in (uxw, uyw, uzw)]
+    I = (slice(1, -1),) * 3
+
+    def d(a, axis, icv):   # (a[+] - a[-]) * 0.5 * ic over the interior, axis in (z, y, x) order
+        hi = [slice(1, -1)] * 3
+        lo = [slice(1, -1)] * 3
+        hi[axis] = slice(2, None)
+        lo[axis] = slice(0, -2)
+        return (a[tuple(hi)] - a[tuple(lo)]) * f32(0.5) * icv
+
+    dux_dx, dux_dy, dux_dz = d(u[0], 2, ic[0]), d(u[0], 1, ic[1]), d(u[0], 0, ic[2])
+    duy_dx, duy_dy, duy_dz = d(u[1], 2, ic[0]), d(u[1], 1, ic[1]), d(u[1], 0, ic[2])
+    duz_dx, duz_dy, duz_dz = d(u[2], 2, ic[0]), d(u[2], 1, ic[1]), d(u[2], 0, ic[2])
+    with np.errstate(all="ignore"):
+        wx, wy, wz = duz_dy - duy_dz, dux_dz - duz_dx, duy_dx - dux_dy
+        vm = np.sqrt(wx * wx + wy * wy + wz * wz)
+        vm = np.where(np.isfinite(vm), vm, f32(0))
+        vm = np.minimum(vm, f32(1000))
+        sxy, sxz, syz = f32(0.5) * (dux_dy + duy_dx), f32(0.5) * (dux_dz + duz_dx), f32(0.5) * (duy_dz + duz_dy)
+        oxy, oxz, oyz = f32(0.5) * (dux_dy - duy_dx), f32(0.5) * (dux_dz - duz_dx), f32(0.5) * (duy_dz - duz_dy)
+        s2 = dux_dx * dux_dx + duy_dy * duy_dy + duz_dz * duz_dz + f32(2) * (sxy * sxy + sxz * sxz + syz * syz)
+        o2 = f32(2) * (oxy * oxy + oxz * oxz + oyz * oyz)
+        q = f32(0.5) * (o2 - s2)
+        q = np.where(np.isfinite(q), q, f32(0))
+        q = np.clip(q, f32(-1e6), f32(1e6))
+        st = np.sqrt(f32(2) * s2)
+        st = np.where(np.isfinite(st), st, f32(0))
+    sol = np.asarray(solid, dtype=bool)[I]
+    good = ~sol & np.isfinite(dux_dx)
+    vort = np.zeros(u[0].shape, f32)
+    qc = np.zeros(u[0].shape, f32)
+    strain = np.full(u[0].shape, np.nan, f32)
+    vort[I] = np.where(good, vm, f32(0))
+    qc[I] = np.where(good, q, f32(0))
+    strain[I] = np.where(good, st, np.nan)
+    return vort, qc, strain
+
+
+def sample_field(field, solid, box_min, cell, p, clamp_density=False):
+    """sampleFieldTrilinear with solid cells skipped (aeros/src/lbm.cpp:1871-1925) at
+    one world point; returns (value as the named sampler returns it, valid)."""
+    nz, ny, nx = field.shape
+    x, y, z = (f32(v) for v in p)
+    default = f32(1) if clamp_density else f32(0)
+    if not (np.isfinite(x) and np.isfinite(y) and np.isfinite(z)):
+        return default, False
+    cs = [f32(c) for c in cell]
+    if min(cs) < f32(1e-9) or nx < 2 or ny < 2 or nz < 2:
+        return default, False
+    fx, fy, fz = (x - f32(box_min[0])) / cs[0], (y - f32(box_min[1])) / cs[1], (z - f32(box_min[2])) / cs[2]
+    ix, iy, iz = int(np.floor(fx)), int(np.floor(fy)), int(np.floor(fz))
+    if ix < 0 or iy < 0 or iz < 0 or ix > nx - 2 or iy > ny - 2 or iz > nz - 2:
+        return default, False
+    cl = lambda t: f32(min(max(t, f32(0)), f32(1)))  # noqa: E731
+    tx, ty, tz = cl(fx - f32(ix)), cl(fy - f32(iy)), cl(fz - f32(iz))
+    wx, wy, wz = (f32(1) - tx, tx), (f32(1) - ty, ty), (f32(1) - tz, tz)
+    acc, wsum = f32(0), f32(0)
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                if solid[iz + dz, iy + dy, ix + dx]:
+                    continue
+                v = f32(field[iz + dz, iy + dy, ix + dx])
+                if not np.isfinite(v):
+                    continue
+                w = wx[dx] * wy[dy] * wz[dz]
+                acc = f32(acc + v * w)
+                wsum = f32(wsum + w)
+    if wsum < f32(1e-6):
+        return default, False
+    acc = f32(acc / wsum)
+    if not np.isfinite(acc):
+        return default, False
+    if clamp_density:
+        acc = f32(min(max(acc, f32(0.8)), f32(1.2)))
+    return acc, True
+
+
+def velocity_lb(ux, uy, uz, solid, box_min, cell, p):
+    """getLBMVelocityLB (aeros/src/lbm.cpp:1781-1843) at one world point."""
+    nz, ny, nx = ux.shape
+    x, y, z = (f32(v) for v in p)
+    zero = np.zeros(3, f32)
+    if not (np.isfinite(x) and np.isfinite(y) and np.isfinite(z)):
+        return zero
+    cs = [f32(c) for c in cell]
+    fx, fy, fz = (x - f32(box_min[0])) / cs[0], (y - f32(box_min[1])) / cs[1], (z - f32(box_min[2])) / cs[2]
+    ix, iy, iz = int(np.floor(fx)), int(np.floor(fy)), int(np.floor(fz))
+    cl = lambda t: f32(min(max(t, f32(0)), f32(1)))  # noqa: E731
+    tx, ty, tz = cl(fx - f32(ix)), cl(fy - f32(iy)), cl(fz - f32(iz))
+    if ix < 0 or ix >= nx - 1 or iy < 0 or iy >= ny - 1 or iz < 0 or iz >= nz - 1:
+        return zero
+    one = f32(1)
+    w = [(one - tx) * (one - ty) * (one - tz), tx * (one - ty) * (one - tz), (one - tx) * ty * (one - tz),
+         tx * ty * (one - tz), (one - tx) * (one - ty) * tz, tx * (one - ty) * tz, (one - tx) * ty * tz, tx * ty * tz]
+    samples, tot = [], f32(0)
+    for k in range(8):
+        cx, cy, cz = ix + (k & 1), iy + ((k >> 1) & 1), iz + ((k >> 2) & 1)
+        if solid[cz, cy, cx]:
+            continue
+        v = np.array([ux[cz, cy, cx], uy[cz, cy, cx], uz[cz, cy, cx]], dtype=f32)
+        if not np.isfinite(v[0]):
+            continue
+        samples.append((v, w[k]))
+        tot = f32(tot + w[k])
+    if not samples:
+        return zero
+    if tot < f32(1e-6):
+        return samples[0][0]
+    r = np.zeros(3, f32)
+    for v, wk in samples:
+        r = (r + v * f32(wk / tot)).astype(f32)
+    return r if np.isfinite(r[0]) else zero
+
+
+def velocity_world(ux, uy, uz, solid, box_min, cell, p, inlet, flow_speed, free_stream):
+    """getLBMVelocityWorld (aeros/src/lbm.cpp:1845-1869) at one world point; inlet is
+    the lattice inflow velocity, free_stream flowParams.v."""
+    nz, ny, nx = ux.shape
+    x, y, z = (f32(v) for v in p)
+    mn = [f32(b) for b in box_min]
+    cs = [f32(c) for c in cell]
+    mx = [mn[0] + f32(nx) * cs[0], mn[1] + f32(ny) * cs[1], mn[2] + f32(nz) * cs[2]]
+    fs = np.array(free_stream, dtype=f32)
+    if x < mn[0] or x > mx[0] or y < mn[1] or y > mx[1] or z < mn[2] or z > mx[2]:
+        return fs
+    if np.isfinite(x) and np.isfinite(y) and np.isfinite(z) and min(cs) >= f32(1e-9):
+        ix = int(np.floor((x - mn[0]) / cs[0]))
+        iy = int(np.floor((y - mn[1]) / cs[1]))
+        iz = int(np.floor((z - mn[2]) / cs[2]))
+        if 0 <= ix < nx and 0 <= iy < ny and 0 <= iz < nz and solid[iz, iy, ix]:
+            return np.zeros(3, f32)
+    v = velocity_lb(ux, uy, uz, solid, box_min, cell, p)
+    if v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < f32(1e-12):
+        return np.zeros(3, f32)
+    a, b, c = (f32(t) for t in inlet)
+    u0 = np.sqrt(a * a + b * b + c * c)
+    if not np.isfinite(u0) or u0 < f32(1e-6):
+        u0 = f32(0.08)
+    scale = f32(flow_speed) / u0
+    if not np.isfinite(scale) or scale > f32(1000):
+        scale = f32(20)
+    w = (v * scale).astype(f32)
+    if not np.isfinite(w[0]):
+        return fs
+    mag = np.sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2])
+    vmax = f32(flow_speed) * f32(2)
+    if mag > vmax:
+        w = (w * (vmax / mag)).astype(f32)
+    return w
