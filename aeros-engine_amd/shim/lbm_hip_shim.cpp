@@ -237,3 +237,33 @@ bool aerosHipControlVolumeForce(float hx, float hy, float hz, glm::vec3* out) {
 bool aerosHipProjectedAreaCells(int axis, float* out) {
     return g_hip && aeros_lbm_projected_area(g_hip, axis, out) == AEROS_OK;
 }
+
+// computeLBMVorticityAndQ (aeros/src/lbm.cpp:2005-2093) on the device, then
+// the three fields it writes into the reference's globals.
+bool aerosHipComputeVorticityAndQ() {
+    if (!g_hip || aeros_lbm_vorticity_q(g_hip) != AEROS_OK) { report("aeros_lbm_vorticity_q"); return false; }
+    const size_t n = (size_t)lbmNx * (size_t)lbmNy * (size_t)lbmNz;
+    pull_field(AEROS_FIELD_VORTICITY, lbmVorticityMag, n);
+    pull_field(AEROS_FIELD_QCRIT, lbmQCriterion, n);
+    pull_field(AEROS_FIELD_STRAIN, lbmStrainMag, n);
+    return true;
+}
+
+// Batched forms of the per-point samplers getLBMPressureWorld /
+// getLBMDensityWorld / getLBMVorticityWorld / getLBMQWorld / getLBMTKEWorld /
+// getLBMStrainWorld and getLBMVelocityWorld (aeros/src/lbm.cpp:1845-1999),
+// for callers that sample many points per frame (streamlines, maps):
+// positions as glm::vec3, results evaluated on the device.
+static_assert(sizeof(glm::vec3) == 3 * sizeof(float), "glm::vec3 must be three packed floats");
+bool aerosHipSampleField(int32_t field, const std::vector<glm::vec3>& pos, std::vector<float>& out) {
+    out.assign(pos.size(), 0.0f);
+    if (!g_hip) return false;
+    return aeros_lbm_sample(g_hip, field, reinterpret_cast<const float*>(pos.data()), (int64_t)pos.size(),
+                            out.data(), nullptr) == AEROS_OK;
+}
+bool aerosHipSampleVelocityWorld(const std::vector<glm::vec3>& pos, std::vector<glm::vec3>& out) {
+    out.assign(pos.size(), glm::vec3(0.0f));
+    if (!g_hip) return false;
+    return aeros_lbm_sample_velocity(g_hip, 1, reinterpret_cast<const float*>(pos.data()), (int64_t)pos.size(),
+                                     reinterpret_cast<float*>(out.data())) == AEROS_OK;
+}
