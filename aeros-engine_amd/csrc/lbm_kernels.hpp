@@ -98,19 +98,6 @@ template <> struct Pop<bf16_t, float> {     // deviation from w_q, bfloat16
     __device__ static void st(bf16_t* p, uint32_t i, float v, float w) { st_nt(at(p, i).bits, f_to_bf16(v - w)); }
 };
 
-// Population q of cell idx in a structure-of-arrays lattice of n cells, by
-// runtime (per-lane) plane index.  For element types up to 4 bytes the whole
-// lattice is addressed from one base with a 32-bit byte offset (the host
-// guarantees 19 n sizeof(S) < 2^32); doubles use a per-direction base.
-template <class S, class R> __device__ inline R ldq(const S* f, uint32_t n, int q, uint32_t idx, R w) {
-    if constexpr (sizeof(S) <= 4) return Pop<S, R>::ld(f, (uint32_t)q * n + idx, w);
-    else return Pop<S, R>::ld(f + (size_t)q * n, idx, w);
-}
-template <class S, class R> __device__ inline void stq(S* f, uint32_t n, int q, uint32_t idx, R v, R w) {
-    if constexpr (sizeof(S) <= 4) Pop<S, R>::st(f, (uint32_t)q * n + idx, v, w);
-    else Pop<S, R>::st(f + (size_t)q * n, idx, v, w);
-}
-
 // A lattice seen by the step kernel: direction planes of n cells, element
 // plane + idx with plane = q n (less the pull shift) >= 0.  Element types up
 // to 4 bytes use one base and a 32-bit byte offset (the host guarantees
@@ -456,7 +443,6 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
         if (blk == 0 && threadIdx.x < kAccSet) B.acc[(size_t)((F.k + 1) % kAccSets) * kAccSet + threadIdx.x] = 0;
     }
 
-    const S* fin_ = static_cast<const S*>(B.f_in);
     const uint32_t n = (uint32_t)P.n;
     const Lat<S> lin(B.f_in), lout(B.f_out);
     const int nx = P.nx, ny = P.ny, nz = P.nz;
@@ -573,20 +559,24 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
                     if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, w_of<R>(opp_of(q)));
             }
         } else {
-            // domain shell across a periodic face: wrap, out-of-domain sources
-            // copy the cell's own population (aeros/src/lbm.cpp:1362-1409)
-            int x, y, z;
-            cell_xyz(P, cell, x, y, z);
+            // domain shell on a periodic face (aeros/src/lbm.cpp:1362-1409): a
+            // source across a periodic face wraps to the opposite face, a
+            // source outside the domain otherwise is replaced by the cell's
+            // own population; in-domain sources pull (or bounce back)
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                int sx = x - cx_of(q), syy = y - cy_of(q), szz = z - cz_of(q);
-                if (P.periodic_x) { if (sx < 0) sx += nx; else if (sx >= nx) sx -= nx; }
-                if (P.periodic_z) { if (szz < 0) szz += nz; else if (szz >= nz) szz -= nz; }
-                const bool out = sx < 0 || sx >= nx || syy < 0 || syy >= ny || szz < 0 || szz >= nz;
-                const bool bb = !out && ((m >> q) & 1u);
-                const uint32_t src = (out || bb) ? cell : ((uint32_t)szz * (uint32_t)ny + (uint32_t)syy) * (uint32_t)nx + (uint32_t)sx;
-                const int dq = bb ? opp_of(q) : q;
-                f[q] = ldq<S, R>(fin_, n, dq, src, w_of<R>(dq));
+                const int off = cx_of(q) + cy_of(q) * nx + cz_of(q) * (int)sz;
+                const uint32_t xf = out_faces(q) & (M_X0 | M_X1), zf = out_faces(q) & (M_Z0 | M_Z1);
+                const bool out = (m & out_faces(q) & ~P.wrap_faces) != 0;
+                const uint32_t wrap = ((m & xf) ? (uint32_t)(cx_of(q) * nx) : 0u) +
+                                      ((m & zf) ? (uint32_t)(cz_of(q) * nz) * sz : 0u);
+                const uint32_t src = out ? cell : cell - (uint32_t)off + wrap;
+                f[q] = lin.ld(src, (uint32_t)q * n, w_of<R>(q));
+            }
+            if (m & M_LINKS) {   // the link bits refer to the wrapped source
+#pragma unroll
+                for (int q = 1; q < Q; ++q)
+                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, w_of<R>(opp_of(q)));
             }
         }
 
@@ -855,21 +845,27 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
                     }
                 }
             } else {
+                // x and z always wrap for this lattice; a source outside in y
+                // bounces back (anti-bounce-back with the wall temperature on
+                // the rows y = 0 and y = Ny-1), as does a solid source
                 int x, y, z;
                 cell_xyz(P, cell, x, y, z);
 #pragma unroll
                 for (int i = 0; i < Q7; ++i) {
-                    int sx = x - cx_of(i), syy = y - cy_of(i), szz = z - cz_of(i);
-                    if (sx < 0) sx = nx - 1; else if (sx >= nx) sx = 0;
-                    if (szz < 0) szz = nz - 1; else if (szz >= nz) szz = 0;
-                    const bool outy = syy < 0 || syy >= ny;
-                    bool bounce = outy;
-                    int wy = syy < 0 ? 0 : ny - 1;
+                    const int off = cx_of(i) + cy_of(i) * nx + cz_of(i) * (int)sz;
+                    const uint32_t xf = out_faces(i) & (M_X0 | M_X1), yf = out_faces(i) & (M_Y0 | M_Y1),
+                                       zf = out_faces(i) & (M_Z0 | M_Z1);
+                    bool bounce;
+                    int wy;
                     uint32_t src = 0;
-                    if (!outy) {
-                        src = ((uint32_t)szz * (uint32_t)ny + (uint32_t)syy) * (uint32_t)nx + (uint32_t)sx;
+                    if (m & yf) {
+                        bounce = true;
+                        wy = cy_of(i) > 0 ? 0 : ny - 1;
+                    } else {
+                        src = cell - (uint32_t)off + ((m & xf) ? (uint32_t)(cx_of(i) * nx) : 0u) +
+                              ((m & zf) ? (uint32_t)(cz_of(i) * nz) * sz : 0u);
                         bounce = (at(B.mask, src) & M_SOLID) != 0;
-                        wy = syy;
+                        wy = y - cy_of(i);
                     }
                     if (bounce) {
                         const R go_ = at(g + (size_t)opp_of(i) * n, cell);
@@ -879,6 +875,7 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
                         gi[i] = at(g + (size_t)i * n, src);
                     }
                 }
+                (void)x; (void)z;
             }
             R T = R(0);
 #pragma unroll
