@@ -224,6 +224,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         P.inlet_face = face_bit(s.axis, s.sign > 0 ? 0 : 1);
         P.outlet_face = face_bit(s.axis, s.sign > 0 ? 1 : 0);
         P.wrap_faces = (c.periodic_x ? (M_X0 | M_X1) : 0u) | (c.periodic_z ? (M_Z0 | M_Z1) : 0u);
+        P.slip_faces = ((uint32_t)c.free_slip & 63u) << M_FACE_SHIFT;
         P.inv_nx = 1.0 / (double)s.nx; P.half_inv_nx = 0.5 / (double)s.nx;
         P.inv_ny = 1.0 / (double)s.ny; P.half_inv_ny = 0.5 / (double)s.ny;
         P.fluid_cells = s.fluid;
@@ -283,7 +284,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         hipLaunchKernelGGL(k_build_mask, dim3(grid_for(n)), dim3(256), 0, st, mask.as<uint32_t>(), dsol.as<uint8_t>(),
                            dgr.as<uint8_t>(), dwall.as<uint8_t>(), s.nx, s.ny, s.nz, s.axis, s.sign,
                            (int)(s.cfg.periodic_x != 0), (int)(s.cfg.periodic_z != 0),
-                           (int)(s.cfg.convective_outlet != 0));
+                           (int)(s.cfg.convective_outlet != 0), ((uint32_t)s.cfg.free_slip & 63u) << M_FACE_SHIFT);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(st));   // the temporaries die at return
         return AEROS_OK;
@@ -845,6 +846,10 @@ aeros_lbm* aeros_lbm_create(const aeros_lbm_config* cfg, int* err) {
         return fail(set_err(AEROS_ERR_ARG, "config struct_size %d != %d", cfg->struct_size, (int)sizeof(aeros_lbm_config)));
     if (cfg->precision < AEROS_FP32 || cfg->precision > AEROS_BF16)
         return fail(set_err(AEROS_ERR_ARG, "unknown precision %d", cfg->precision));
+    {
+        char msg[160];
+        if (check_boundaries(*cfg, msg, (int)sizeof msg)) return fail(set_err(AEROS_ERR_ARG, "%s", msg));
+    }
     int rc = check_device(cfg->device);
     if (rc) return fail(rc);
     std::unique_ptr<aeros_lbm> h(new (std::nothrow) aeros_lbm());

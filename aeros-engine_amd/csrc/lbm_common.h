@@ -15,7 +15,8 @@ constexpr int Q7 = 7;   // D3Q7 (first seven D3Q19 directions)
 
 // Per-cell link mask (uint32), built once per mask:
 //   bits 0..18  source of direction q is solid/ground (half-way bounce-back);
-//               for shell cells the bit refers to the wrapped in-domain source
+//               for shell cells the bit refers to the in-domain source after a
+//               periodic wrap or a free-slip reflection
 //   bit  19     the cell itself is solid or ground (never updated)
 //   bits 20..22 Van Driest table index: wall distance 0..5, 6 = no damping
 //   bit  23     Smagorinsky strain is defined here (interior, not on the inlet
@@ -58,6 +59,9 @@ struct Setup {
     bool curl_noise = false;                    // inletTurbulence > 0.001
 };
 
+// Free-slip faces must be known faces off the periodic axes, and cannot be
+// combined with the thermal lattice: AEROS_OK or AEROS_ERR_ARG with a reason.
+int check_boundaries(const aeros_lbm_config& cfg, char* msg, int msglen);
 // initLBM's host half: returns AEROS_OK or an error code; msg gets the reason.
 int setup_domain(const aeros_lbm_config& cfg, Setup& s, char* msg, int msglen);
 // Everything after the mask is known: wall distance, fluid count, tau, thermal.
@@ -121,9 +125,10 @@ template <class R> struct StepParams {
     R t_inv_tau, t_ref, t_g, t_hi, t_lo;
     R t_hot, t_cold;
     int t_fixed;
-    // domain faces: the inlet and outlet planes, and the faces a periodic
-    // axis wraps (their shell cells take the general pull)
-    uint32_t inlet_face, outlet_face, wrap_faces;
+    // domain faces: the inlet and outlet planes, the faces a periodic axis
+    // wraps and the free-slip faces (shell cells on either of the last two
+    // take the general pull)
+    uint32_t inlet_face, outlet_face, wrap_faces, slip_faces;
     // cell -> (x, y, z): floor(c / d) = (uint)(c inv_d + half_inv_d) in FP64,
     // exact for c < 2^26 and d < 2^25
     double inv_nx, half_inv_nx, inv_ny, half_inv_ny;

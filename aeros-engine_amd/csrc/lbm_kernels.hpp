@@ -125,6 +125,39 @@ __device__ __host__ constexpr uint32_t out_faces(int q) {
            (cy_of(q) < 0 ? M_Y1 : 0u) | (cz_of(q) > 0 ? M_Z0 : 0u) | (cz_of(q) < 0 ? M_Z1 : 0u);
 }
 
+// Index of the direction with velocity (x, y, z); -1 if there is none.
+__device__ __host__ constexpr int dir_of(int x, int y, int z) {
+    for (int q = 0; q < Q; ++q)
+        if (cx_of(q) == x && cy_of(q) == y && cz_of(q) == z) return q;
+    return -1;
+}
+// Direction q with its components along the axes in `axes` (bit a = axis a)
+// negated: the direction after a specular reflection off those walls.
+__device__ __host__ constexpr int mirror_of(int q, int axes) {
+    return dir_of(axes & 1 ? -cx_of(q) : cx_of(q), axes & 2 ? -cy_of(q) : cy_of(q), axes & 4 ? -cz_of(q) : cz_of(q));
+}
+
+// Source of the pull of direction q into cell (x, y, z).  Beyond a free-slip
+// face (a specular wall half-way outside it) the source is reflected: the
+// population left the cell's mirror image across the wall, moving in the
+// mirrored direction dq.  Beyond a periodic face it wraps.  Returns false
+// when the source lies beyond any other face: the cell then keeps its own
+// population (aeros/src/lbm.cpp:1392-1394).
+__device__ __host__ inline bool pull_source(int x, int y, int z, int q, int nx, int ny, int nz, int periodic_x,
+                                            int periodic_z, uint32_t slip, int& sx, int& sy, int& sz, int& dq) {
+    sx = x - cx_of(q);
+    sy = y - cy_of(q);
+    sz = z - cz_of(q);
+    int axes = 0;
+    if ((sx < 0 && (slip & M_X0)) || (sx >= nx && (slip & M_X1))) { sx = x; axes |= 1; }
+    if ((sy < 0 && (slip & M_Y0)) || (sy >= ny && (slip & M_Y1))) { sy = y; axes |= 2; }
+    if ((sz < 0 && (slip & M_Z0)) || (sz >= nz && (slip & M_Z1))) { sz = z; axes |= 4; }
+    if (periodic_x) { if (sx < 0) sx += nx; else if (sx >= nx) sx -= nx; }
+    if (periodic_z) { if (sz < 0) sz += nz; else if (sz >= nz) sz -= nz; }
+    dq = mirror_of(q, axes);
+    return sx >= 0 && sx < nx && sy >= 0 && sy < ny && sz >= 0 && sz < nz;
+}
+
 // Cell index -> coordinates without an integer division.
 template <class R> __device__ inline void cell_xyz(const StepParams<R>& P, uint32_t c, int& x, int& y, int& z) {
     const uint32_t t = (uint32_t)__builtin_fma((double)c, P.inv_nx, P.half_inv_nx);   // c / nx
@@ -423,7 +456,7 @@ struct StepFlags {
 
 // One lattice step over blocks blk of nblk (the whole grid for a single
 // lattice, the lattice's share of the launch in served mode).
-template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
+template <class R, class S, class U, bool LES, bool THERM, bool FORCE, bool SLIP = false>
 __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuffers& B, const StepFlags F,
                                           const uint32_t blk, const uint32_t nblk, RedLds& L) {
     Status* st = B.status;
@@ -543,8 +576,8 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
                 const R fi2 = lin.ld(ci2, (uint32_t)q * n, w);
                 f[q] = fo + P.conv_c * (fi - fo) + P.conv_k2 * (fi - fi2);
             }
-        } else if (!(m & P.wrap_faces)) {
-            // domain shell with no periodic wrap: a source outside the domain
+        } else if (!(m & (SLIP ? (P.wrap_faces | P.slip_faces) : P.wrap_faces))) {
+            // domain shell with no periodic wrap or free-slip wall: a source outside the domain
             // is replaced by the cell's own population (aeros/src/lbm.cpp:1392-1394);
             // in-domain sources pull (or bounce back) as in the interior
 #pragma unroll
@@ -559,21 +592,45 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
                     if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, w_of<R>(opp_of(q)));
             }
         } else {
-            // domain shell on a periodic face (aeros/src/lbm.cpp:1362-1409): a
-            // source across a periodic face wraps to the opposite face, a
-            // source outside the domain otherwise is replaced by the cell's
-            // own population; in-domain sources pull (or bounce back)
+            // domain shell on a periodic or free-slip face: a source beyond a
+            // periodic face wraps to the opposite face (aeros/src/lbm.cpp:1384-1391),
+            // one beyond a free-slip face is read at the cell's mirror image in
+            // the mirrored direction (specular reflection), one beyond any
+            // other face is replaced by the cell's own population; in-domain
+            // sources pull (or bounce back) as in the interior
+            if constexpr (SLIP) {
+                const uint32_t special = P.wrap_faces | P.slip_faces;
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                const int off = cx_of(q) + cy_of(q) * nx + cz_of(q) * (int)sz;
-                const uint32_t xf = out_faces(q) & (M_X0 | M_X1), zf = out_faces(q) & (M_Z0 | M_Z1);
-                const bool out = (m & out_faces(q) & ~P.wrap_faces) != 0;
-                const uint32_t wrap = ((m & xf) ? (uint32_t)(cx_of(q) * nx) : 0u) +
-                                      ((m & zf) ? (uint32_t)(cz_of(q) * nz) * sz : 0u);
-                const uint32_t src = out ? cell : cell - (uint32_t)off + wrap;
-                f[q] = lin.ld(src, (uint32_t)q * n, w_of<R>(q));
+                for (int q = 0; q < Q; ++q) {
+                    const int off = cx_of(q) + cy_of(q) * nx + cz_of(q) * (int)sz;
+                    const uint32_t of = m & out_faces(q), wr = of & P.wrap_faces, rf = of & P.slip_faces;
+                    const bool out = (of & ~special) != 0;
+                    const bool rx = (rf & (M_X0 | M_X1)) != 0, ry = (rf & (M_Y0 | M_Y1)) != 0,
+                               rz = (rf & (M_Z0 | M_Z1)) != 0;
+                    const uint32_t src = cell - (uint32_t)off + ((wr & (M_X0 | M_X1)) ? (uint32_t)(cx_of(q) * nx) : 0u) +
+                                         ((wr & (M_Z0 | M_Z1)) ? (uint32_t)(cz_of(q) * nz) * sz : 0u) +
+                                         (rx ? (uint32_t)cx_of(q) : 0u) + (ry ? (uint32_t)(cy_of(q) * nx) : 0u) +
+                                         (rz ? (uint32_t)cz_of(q) * sz : 0u);
+                    // a D3Q19 velocity has at most two non-zero components, and
+                    // only those can be reflected: two reflections give opp(q)
+                    const int dq = ((int)rx + (int)ry + (int)rz >= 2)
+                                       ? opp_of(q)
+                                       : (rx ? mirror_of(q, 1) : (ry ? mirror_of(q, 2) : (rz ? mirror_of(q, 4) : q)));
+                    f[q] = lin.ld(out ? cell : src, (uint32_t)(out ? q : dq) * n, w_of<R>(q));
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    const int off = cx_of(q) + cy_of(q) * nx + cz_of(q) * (int)sz;
+                    const uint32_t xf = out_faces(q) & (M_X0 | M_X1), zf = out_faces(q) & (M_Z0 | M_Z1);
+                    const bool out = (m & out_faces(q) & ~P.wrap_faces) != 0;
+                    const uint32_t wrap = ((m & xf) ? (uint32_t)(cx_of(q) * nx) : 0u) +
+                                          ((m & zf) ? (uint32_t)(cz_of(q) * nz) * sz : 0u);
+                    const uint32_t src = out ? cell : cell - (uint32_t)off + wrap;
+                    f[q] = lin.ld(src, (uint32_t)q * n, w_of<R>(q));
+                }
             }
-            if (m & M_LINKS) {   // the link bits refer to the wrapped source
+            if (m & M_LINKS) {   // the link bits refer to the wrapped or reflected source
 #pragma unroll
                 for (int q = 1; q < Q; ++q)
                     if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, w_of<R>(opp_of(q)));
@@ -926,11 +983,13 @@ __global__ __launch_bounds__(BLOCK) void k_fold(StepParams<R> P, StepBuffers B, 
     fold_last<R>(P, B, k, nblk, L);
 }
 
-template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
+// SLIP: some domain face carries a free-slip wall (a separate instance, so
+// that the reflection's addressing costs the other lattices no registers).
+template <class R, class S, class U, bool LES, bool THERM, bool FORCE, bool SLIP = false>
 __global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) {
     __shared__ RedLds L;
-    step_grid<R, S, U, LES, THERM, FORCE>(P, B, StepFlags{P.first, P.write_fields, P.dry, P.k, P.pending}, blockIdx.x,
-                                          gridDim.x, L);
+    step_grid<R, S, U, LES, THERM, FORCE, SLIP>(P, B, StepFlags{P.first, P.write_fields, P.dry, P.k, P.pending},
+                                                blockIdx.x, gridDim.x, L);
 }
 
 // ------------------------------------------------------------ served mode
@@ -1013,7 +1072,8 @@ __global__ void k_reset(S* f, const uint32_t* mask, int64_t n, StepParams<R> P, 
 
 // Per-cell link masks from the solid/ground/wall-distance bytes.
 __global__ void k_build_mask(uint32_t* mask, const uint8_t* solid, const uint8_t* ground, const uint8_t* wall, int nx,
-                             int ny, int nz, int axis, int sign, int periodic_x, int periodic_z, int conv_outlet) {
+                             int ny, int nz, int axis, int sign, int periodic_x, int periodic_z, int conv_outlet,
+                             uint32_t slip) {
     const int64_t n = (int64_t)nx * ny * nz;
     const int64_t sy = nx, sz = (int64_t)nx * ny;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
@@ -1035,10 +1095,8 @@ __global__ void k_build_mask(uint32_t* mask, const uint8_t* solid, const uint8_t
             if (inl && !periodic_x) m |= M_PULL_INLET;
             else if (outl && conv_outlet && !periodic_x) m |= M_PULL_OUTLET;
             for (int q = 0; q < Q; ++q) {
-                int sx = x - cx_of(q), syy = y - cy_of(q), szz = z - cz_of(q);
-                if (periodic_x) { if (sx < 0) sx += nx; else if (sx >= nx) sx -= nx; }
-                if (periodic_z) { if (szz < 0) szz += nz; else if (szz >= nz) szz -= nz; }
-                if (sx < 0 || sx >= nx || syy < 0 || syy >= ny || szz < 0 || szz >= nz) continue;
+                int sx, syy, szz, dq;
+                if (!pull_source(x, y, z, q, nx, ny, nz, periodic_x, periodic_z, slip, sx, syy, szz, dq)) continue;
                 const int64_t s = ((int64_t)szz * ny + syy) * nx + sx;
                 if (solid[s] || ground[s]) m |= 1u << q;
             }
@@ -1086,13 +1144,11 @@ __global__ void k_pull_only(const S* fin_, const S* fsolid, R* out, const uint32
         }
         const int x = (int)(cell % nx), y = (int)((cell / nx) % ny), z = (int)(cell / sz);
         for (int q = 0; q < Q; ++q) {
-            int sx = x - cx_of(q), syy = y - cy_of(q), szz = z - cz_of(q);
-            if (P.periodic_x) { if (sx < 0) sx += nx; else if (sx >= nx) sx -= nx; }
-            if (P.periodic_z) { if (szz < 0) szz += nz; else if (szz >= nz) szz -= nz; }
-            const bool out_ = sx < 0 || sx >= nx || syy < 0 || syy >= ny || szz < 0 || szz >= nz;
-            const bool bb = !out_ && ((m >> q) & 1u);
-            const int64_t src = (out_ || bb) ? cell : ((int64_t)szz * ny + syy) * nx + sx;
-            const int dq = bb ? opp_of(q) : q;
+            int sx, syy, szz, mq;
+            const bool in = pull_source(x, y, z, q, nx, ny, nz, P.periodic_x, P.periodic_z, P.slip_faces, sx, syy, szz, mq);
+            const bool bb = in && ((m >> q) & 1u);
+            const int64_t src = (!in || bb) ? cell : ((int64_t)szz * ny + syy) * nx + sx;
+            const int dq = bb ? opp_of(q) : (in ? mq : q);
             out[q * n + cell] = Pop<S, R>::ld(fin_ + (size_t)dq * n, (uint32_t)src, w_of<R>(dq));
         }
         (void)sy;
@@ -1607,15 +1663,22 @@ template <class T> __global__ void k_fill(T* p, int64_t n, T v) {
 }
 
 // ------------------------------------------------------------ launchers
-template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
+template <class R, class S, class U, bool LES, bool THERM, bool FORCE, bool SLIP = false>
 static hipError_t launch_step_t(const StepParams<R>& P, const StepBuffers& B, int grid, hipStream_t s) {
-    hipLaunchKernelGGL((k_step<R, S, U, LES, THERM, FORCE>), dim3(grid), dim3(BLOCK), 0, s, P, B);
+    hipLaunchKernelGGL((k_step<R, S, U, LES, THERM, FORCE, SLIP>), dim3(grid), dim3(BLOCK), 0, s, P, B);
     return hipGetLastError();
 }
 
 template <class R, class S, class U>
 hipError_t launch_step(const StepParams<R>& P, const StepBuffers& B, int grid, hipStream_t s) {
     const bool force = P.fx != R(0) || P.fy != R(0) || P.fz != R(0);
+    if (P.slip_faces) {   // the set-up rejects free-slip walls together with the thermal lattice
+        if (P.thermal) return hipErrorInvalidValue;
+        if (P.les) return force ? launch_step_t<R, S, U, true, false, true, true>(P, B, grid, s)
+                                : launch_step_t<R, S, U, true, false, false, true>(P, B, grid, s);
+        return force ? launch_step_t<R, S, U, false, false, true, true>(P, B, grid, s)
+                     : launch_step_t<R, S, U, false, false, false, true>(P, B, grid, s);
+    }
     if (P.les) {
         if (P.thermal) return force ? launch_step_t<R, S, U, true, true, true>(P, B, grid, s)
                                     : launch_step_t<R, S, U, true, true, false>(P, B, grid, s);

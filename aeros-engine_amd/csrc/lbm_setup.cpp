@@ -105,10 +105,24 @@ void flow_axis(const aeros_lbm_config& c, int& axis, int& sign) {
     sign = r.d[axis] > 0 ? 1 : -1;
 }
 
+int check_boundaries(const aeros_lbm_config& c, char* msg, int msglen) {
+    if ((c.free_slip & ~63) || (c.periodic_x && (c.free_slip & (AEROS_FACE_X0 | AEROS_FACE_X1))) ||
+        (c.periodic_z && (c.free_slip & (AEROS_FACE_Z0 | AEROS_FACE_Z1)))) {
+        set_msg(msg, msglen, "free_slip names an unknown face or a face of a periodic axis");
+        return AEROS_ERR_ARG;
+    }
+    if (c.free_slip && c.thermal) {   // the D3Q7 lattice keeps its own shell rules (x and z wrap)
+        set_msg(msg, msglen, "free-slip walls are not available together with the thermal lattice");
+        return AEROS_ERR_ARG;
+    }
+    return AEROS_OK;
+}
+
 int setup_domain(const aeros_lbm_config& cfg, Setup& s, char* msg, int msglen) {
     s = Setup{};
     s.cfg = cfg;
     const aeros_lbm_config& c = cfg;
+    if (int rc = check_boundaries(c, msg, msglen)) return rc;
     const bool vox = c.voxels && c.vox_n[0] > 0 && c.vox_n[1] > 0 && c.vox_n[2] > 0;
     const int64_t cap_pad = c.cell_cap > 0 ? c.cell_cap : 8LL * 1024 * 1024;    // aeros/src/lbm.cpp:467
     const int64_t cap_hard = c.cell_cap > 0 ? c.cell_cap : 20LL * 1024 * 1024;  // aeros/src/lbm.cpp:558

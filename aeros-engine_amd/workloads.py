@@ -131,14 +131,29 @@ def lattice_box(nx, ny, nz, cell=1.0):
     return dict(vox_min=(0.0, 0.0, 0.0), vox_max=(nx * cell, ny * cell, nz * cell), domain_padding=0.0)
 
 
-def c1_sphere_fp64() -> dict:
-    """96x48x48, sphere D=12 at (24,24,24), U0 0.05 along +x, BGK, tau = 0.5 + 3 U0 D / Re (Re 100)."""
+SIDE_WALLS = 4 | 8 | 16 | 32   # free-slip on y = 0, y = Ny-1, z = 0, z = Nz-1
+
+
+def c1_sphere_fp64(precision="fp64") -> dict:
+    """96x48x48, sphere D=12 at (24,24,24), U0 0.05 along +x, BGK, tau = 0.5 + 3 U0 D / Re (Re 100),
+    uniform inflow, zero-gradient outflow (copy in place), free-slip side walls (y and z faces)."""
     nx, ny, nz = 96, 48, 48
     vox = sphere_in_box(nx, ny, nz, (24, 24, 24), 12)
     tau = float(f32(0.5) + f32(3.0) * f32(0.05) * f32(12.0) / f32(100.0))
     return dict(voxels=vox, **lattice_box(nx, ny, nz), voxel_resolution=nx, max_dim=12.0,
                 flow_speed=20.0, U0=0.05, tau=tau, use_les=0, use_trt=0, auto_viscosity=0,
-                inlet_turbulence=0.0, convective_outlet=0, precision="fp64")
+                inlet_turbulence=0.0, convective_outlet=0, free_slip=SIDE_WALLS, precision=precision)
+
+
+def slip_box(nx, ny, nz, centre, diameter, free_slip, U0=0.06, tau=0.56, **over) -> dict:
+    """Small sphere-in-a-box case for the free-slip walls: lattice-aligned box, BGK, no LES,
+    uniform inflow along +x, zero-gradient outflow."""
+    vox = sphere_in_box(nx, ny, nz, centre, diameter)
+    cfg = dict(voxels=vox, **lattice_box(nx, ny, nz), voxel_resolution=nx, max_dim=float(diameter),
+               flow_speed=10.0, U0=U0, tau=tau, use_les=0, use_trt=0, auto_viscosity=0,
+               inlet_turbulence=0.0, convective_outlet=0, free_slip=free_slip)
+    cfg.update(over)
+    return cfg
 
 
 def ellipsoid_mesh(semi=(30.0, 12.0, 12.0), centre=(0.0, 0.0, 0.0), n_u=316, n_v=316,

@@ -75,6 +75,22 @@ def run_gpu(A, cfg, steps, warmup, label, T=None):
     return res
 
 
+def c1_results(A, W):
+    """The C1 sphere's reported results: the FP64 rho sum after 120 steps and
+    Cd after 2000 steps (control volume 0.45/0.40/0.40 of the domain around its
+    centre, which holds the sphere; A = projected solid cells along x)."""
+    L = A.Lattice(A.default_config(**W.c1_sphere_fp64()))
+    L.step(120)
+    rho_sum, _ = L.checksums()
+    L.step(1880)
+    fx, _, _ = L.cv_force(0.45, 0.40, 0.40)
+    area = L.projected_area(0)
+    cd = fx / (0.5 * 0.05 * 0.05 * area)
+    L.close()
+    log(f"[c1 fp64] rho sum after 120 steps {rho_sum:.12g}; Cd after 2000 steps {cd:.4f} (A {area:.0f} cells)")
+    return {"rho_sum_120": float("%.12g" % rho_sum), "cd_2000": round(cd, 4), "area_cells": area}
+
+
 def line(r, bpc):
     return {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
             "pct_hbm": round(r["mlups"] * 1e6 * bpc / (HBM_PEAK_GBS * 1e9), 4),
@@ -320,10 +336,12 @@ def main():
         sec["fp16_over_fp32"] = round(sec["c4_fp16"]["mlups"] / sec["c4_fp32"]["mlups"], 3)
         sec["bf16_over_fp32"] = round(sec["c4_bf16"]["mlups"] / sec["c4_fp32"]["mlups"], 3)
         sec["c1_fp64"] = line(run_gpu(A, W.c1_sphere_fp64(), 1000, 50, "c1 fp64"), BYTES_PER_CELL[("fp64", False)])
+        sec["c1_fp64"].update(c1_results(A, W))
         c3cfg, c3T = W.c3_rayleigh_benard()
         sec["c3_thermal_fp32"] = line(run_gpu(A, c3cfg, 500, 20, "c3 thermal", T=c3T), BYTES_PER_CELL[("thermal", False)])
         sec["workloads"] = {"c4": "512x256x256 (33.5M cells), ellipsoid scaled x2, Re 1000, TRT+LES",
-                            "c1": "96x48x48 FP64 BGK sphere D=12, U0 0.05, zero-gradient outflow",
+                            "c1": "96x48x48 FP64 BGK sphere D=12, U0 0.05, zero-gradient outflow, "
+                                  "free-slip side walls",
                             "c3": "128x64x128 coupled D3Q19+D3Q7 FP32, Rayleigh-Benard Ra 2500 Pr 0.71"}
         out["secondary"] = sec
     if not args.no_served:

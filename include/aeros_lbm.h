@@ -132,7 +132,19 @@ typedef struct aeros_lbm_config {
     int64_t cell_cap;             /* 0 = the reference's caps (8 Mi padded, 20 Mi hard);
                                      >0 = explicit cap on Nx*Ny*Nz                          */
     int32_t device;               /* HIP device ordinal                                    */
+    int32_t free_slip;            /* faces closed by a free-slip (specular) wall, a mask of
+                                     AEROS_FACE_*; none may lie on a periodic axis.  No
+                                     counterpart in the reference, whose non-periodic faces
+                                     copy the cell's own population in place.              */
 } aeros_lbm_config;
+
+/* Domain faces (aeros_lbm_config.free_slip). */
+#define AEROS_FACE_X0 1   /* x = 0      */
+#define AEROS_FACE_X1 2   /* x = Nx - 1 */
+#define AEROS_FACE_Y0 4   /* y = 0      */
+#define AEROS_FACE_Y1 8   /* y = Ny - 1 */
+#define AEROS_FACE_Z0 16  /* z = 0      */
+#define AEROS_FACE_Z1 32  /* z = Nz - 1 */
 
 /* Scalars the reference keeps in globals (aeros/src/lbm.h:77-125,
  * aeros/src/thermal_lbm.h:46-58).  Values are those after the last step. */

@@ -3,7 +3,8 @@
 //
 // A restatement, in our own code, of the reference solver's algorithm:
 // D3Q19 collide + pull stream (BGK/TRT, Guo forcing, Smagorinsky LES with Van
-// Driest damping, half-way bounce-back, periodic / copy-in-place shell,
+// Driest damping, half-way bounce-back, periodic / copy-in-place shell (and
+// free-slip walls, which the reference lacks),
 // equilibrium inlet, convective outlet), diagnostics, divergence reset and the
 // end-of-call post-pass; the D3Q7 thermal lattice with Boussinesq buoyancy and
 // Nusselt number; the control-volume drag and projected area; and the BVH
@@ -1021,13 +1022,23 @@ public:
                     if (solid[d] || ground[d]) continue;
                     for (int q = 0; q < Q; ++q) {
                         int sx = x - CV[q][0], sy = y - CV[q][1], sz = z - CV[q][2];
+                        // free-slip wall half a cell outside a face (no reference
+                        // counterpart): the population arriving here left this
+                        // cell's mirror image, moving in the mirrored direction
+                        int c[3] = {CV[q][0], CV[q][1], CV[q][2]};
+                        if ((sx < 0 && (cfg.free_slip & AEROS_FACE_X0)) || (sx >= Nx && (cfg.free_slip & AEROS_FACE_X1))) { sx = x; c[0] = -c[0]; }
+                        if ((sy < 0 && (cfg.free_slip & AEROS_FACE_Y0)) || (sy >= Ny && (cfg.free_slip & AEROS_FACE_Y1))) { sy = y; c[1] = -c[1]; }
+                        if ((sz < 0 && (cfg.free_slip & AEROS_FACE_Z0)) || (sz >= Nz && (cfg.free_slip & AEROS_FACE_Z1))) { sz = z; c[2] = -c[2]; }
+                        int qs = q;
+                        for (int k = 0; k < Q; ++k)
+                            if (CV[k][0] == c[0] && CV[k][1] == c[1] && CV[k][2] == c[2]) qs = k;
                         if (cfg.periodic_x) { if (sx < 0) sx += Nx; else if (sx >= Nx) sx -= Nx; }
                         if (cfg.periodic_z) { if (sz < 0) sz += Nz; else if (sz >= Nz) sz -= Nz; }
                         if (sx < 0 || sx >= Nx || sy < 0 || sy >= Ny || sz < 0 || sz >= Nz)
                             fNext[(size_t)d * Q + q] = f[(size_t)d * Q + q];
                         else {
                             const int s = idx(sx, sy, sz);
-                            fNext[(size_t)d * Q + q] = (solid[s] || ground[s]) ? f[(size_t)d * Q + OPP[q]] : f[(size_t)s * Q + q];
+                            fNext[(size_t)d * Q + q] = (solid[s] || ground[s]) ? f[(size_t)d * Q + OPP[q]] : f[(size_t)s * Q + qs];
                         }
                     }
                 }
