@@ -113,6 +113,26 @@ class Scalars(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class PhysClockState(C.Structure):
+    """aeros_phys_clock: the fixed-tick physics clock (aeros/src/phys_clock.h:48-250)."""
+    _fields_ = [
+        ("fixed", C.c_int32), ("hz", C.c_float), ("max_substeps", C.c_int32),
+        ("particle_substeps", C.c_int32), ("max_frame_dt", C.c_float),
+        ("accumulator", C.c_float), ("lattice_accum", C.c_float),
+        ("sim_time", C.c_double), ("real_time", C.c_double), ("dropped_time", C.c_double),
+        ("total_ticks", C.c_int64), ("total_lattice_steps", C.c_int64),
+        ("ticks", C.c_int32), ("lattice_steps", C.c_int32), ("part_steps", C.c_int32),
+        ("tick_dt", C.c_float), ("achieved_hz", C.c_float), ("time_scale", C.c_float),
+        ("wall_scale", C.c_float), ("display_ratio", C.c_float), ("saturated", C.c_int32),
+    ]
+
+
+class Frame(C.Structure):
+    """aeros_frame: one finished asynchronous frame."""
+    _fields_ = [("frame", C.c_int64), ("steps", C.c_int32), ("scalars", Scalars),
+                ("field", C.POINTER(C.c_float) * 17)]
+
+
 class BackendInfo(C.Structure):
     _fields_ = [
         ("compiled_with_hip", C.c_int32), ("device_count", C.c_int32), ("usable", C.c_int32),
@@ -251,6 +271,19 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
         "aeros_lbm_set_mask": (C.c_int, [vp, P(C.c_uint8), C.c_int32, C.c_int32, C.c_int32]),
         "aeros_lbm_step": (C.c_int, [vp, C.c_int32]),
         "aeros_lbm_update": (C.c_int, [vp, C.c_float, C.c_int32, C.c_int32]),
+        "aeros_lbm_update_steps": (C.c_int32, [C.c_float, C.c_int32, C.c_int32]),
+        "aeros_lbm_lattice_dt": (C.c_int, [vp, P(C.c_float)]),
+        "aeros_clock_defaults": (None, [P(PhysClockState)]),
+        "aeros_clock_reset": (None, [P(PhysClockState)]),
+        "aeros_clock_sanitize": (None, [P(PhysClockState)]),
+        "aeros_clock_advance": (C.c_int32, [P(PhysClockState), C.c_float, C.c_float, C.c_float, P(C.c_int32)]),
+        "aeros_clock_particle_dt": (C.c_float, [P(PhysClockState)]),
+        "aeros_clock_frame_sim_dt": (C.c_float, [P(PhysClockState)]),
+        "aeros_lbm_frames_start": (C.c_int, [vp, C.c_uint32]),
+        "aeros_lbm_frame_submit": (C.c_int, [vp, C.c_float, C.c_int32, C.c_int32]),
+        "aeros_lbm_frame_latest": (C.c_int, [vp, P(Frame)]),
+        "aeros_lbm_frames_wait": (C.c_int, [vp]),
+        "aeros_lbm_frames_stop": (C.c_int, [vp]),
         "aeros_lbm_read_field": (C.c_int, [vp, C.c_int32, P(C.c_float)]),
         "aeros_lbm_read_field_f64": (C.c_int, [vp, C.c_int32, P(C.c_double)]),
         "aeros_lbm_read": (C.c_int, [vp, C.c_uint32, P(C.c_float), P(C.c_float), P(C.c_float),
@@ -317,6 +350,49 @@ def last_error() -> str:
 
 class AerosError(RuntimeError):
     pass
+
+
+class PhysClock:
+    """The fixed-tick physics clock of the host stepping driver (host code)."""
+
+    def __init__(self, **settings):
+        self.lib = load_library()
+        self.c = PhysClockState()
+        self.lib.aeros_clock_defaults(C.byref(self.c))
+        for k, v in settings.items():
+            setattr(self.c, k, v)
+
+    def __getattr__(self, k):
+        if k in ("lib", "c"):
+            raise AttributeError(k)
+        return getattr(self.c, k)
+
+    def set(self, **settings):
+        for k, v in settings.items():
+            setattr(self.c, k, v)
+
+    def advance(self, frame_dt: float, lattice_dt: float = 0.0, fps: float = 60.0):
+        """Returns (ticks, lattice steps) for one rendered frame."""
+        lbm = C.c_int32(0)
+        t = self.lib.aeros_clock_advance(C.byref(self.c), float(frame_dt), float(lattice_dt), float(fps), C.byref(lbm))
+        return t, lbm.value
+
+    def sanitize(self):
+        self.lib.aeros_clock_sanitize(C.byref(self.c))
+
+    def reset(self):
+        self.lib.aeros_clock_reset(C.byref(self.c))
+
+    def particle_dt(self) -> float:
+        return self.lib.aeros_clock_particle_dt(C.byref(self.c))
+
+    def frame_sim_dt(self) -> float:
+        return self.lib.aeros_clock_frame_sim_dt(C.byref(self.c))
+
+
+def update_steps(dt: float, force_steps: int = -1, steps_per_frame: int = 3) -> int:
+    """updateLBM's step count for these arguments (host rule)."""
+    return load_library().aeros_lbm_update_steps(float(dt), int(force_steps), int(steps_per_frame))
 
 
 def _check(rc: int, what: str) -> None:
@@ -389,6 +465,42 @@ class Lattice:
     def update(self, dt: float, force_steps: int = -1, steps_per_frame: int = 3):
         _check(self.lib.aeros_lbm_update(self.h, float(dt), int(force_steps), int(steps_per_frame)),
                "aeros_lbm_update")
+
+    def lattice_dt(self) -> float:
+        v = C.c_float()
+        _check(self.lib.aeros_lbm_lattice_dt(self.h, C.byref(v)), "aeros_lbm_lattice_dt")
+        return v.value
+
+    # asynchronous frames (host stepping driver)
+    def frames_start(self, fields=("rho", "ux", "uy", "uz")):
+        mask = 0
+        for f in fields:
+            mask |= 1 << (FIELD_NAMES[f] if isinstance(f, str) else int(f))
+        self._frame_fields = [FIELD_NAMES[f] if isinstance(f, str) else int(f) for f in fields]
+        _check(self.lib.aeros_lbm_frames_start(self.h, mask), "aeros_lbm_frames_start")
+
+    def frame_submit(self, dt: float, force_steps: int = -1, steps_per_frame: int = 3):
+        _check(self.lib.aeros_lbm_frame_submit(self.h, float(dt), int(force_steps), int(steps_per_frame)),
+               "aeros_lbm_frame_submit")
+
+    def frame_latest(self, copy: bool = True) -> dict:
+        """Newest finished frame: {'frame', 'steps', 'scalars', 'fields': {name: array}}.
+        copy=False returns views of the pinned slot (valid until the next call)."""
+        fr = Frame()
+        _check(self.lib.aeros_lbm_frame_latest(self.h, C.byref(fr)), "aeros_lbm_frame_latest")
+        names = {v: k for k, v in FIELD_NAMES.items()}
+        out = {"frame": fr.frame, "steps": fr.steps, "scalars": fr.scalars.as_dict(), "fields": {}}
+        if fr.frame:
+            for fid in self._frame_fields:
+                a = np.ctypeslib.as_array(fr.field[fid], shape=(self.ncells,)).reshape(self.shape)
+                out["fields"][names[fid]] = a.copy() if copy else a
+        return out
+
+    def frames_wait(self):
+        _check(self.lib.aeros_lbm_frames_wait(self.h), "aeros_lbm_frames_wait")
+
+    def frames_stop(self):
+        _check(self.lib.aeros_lbm_frames_stop(self.h), "aeros_lbm_frames_stop")
 
     def reset(self):
         _check(self.lib.aeros_lbm_reset(self.h), "aeros_lbm_reset")

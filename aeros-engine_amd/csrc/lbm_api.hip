@@ -12,6 +12,10 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -106,12 +110,17 @@ struct EngineBase {
     virtual int vorticity_q() = 0;
     virtual int sample(int field, const float* pos, int64_t np, float* out, uint8_t* valid) = 0;
     virtual int sample_velocity(int world, const float* pos, int64_t np, float* out) = 0;
+    // A field, as float, into a device buffer on the solver stream (no wait).
+    virtual int field_device(int id, float* dst) = 0;
+    virtual int step_count() const = 0;   // lbmCurrentStep
     // HIP-event timing of the step kernel launches
     bool timing = false;
     double timed_ms = 0.0;
     int64_t timed_launches = 0;
     std::vector<hipEvent_t> events;
 };
+
+struct FrameDriver;
 
 struct aeros_lbm {
     aeros_lbm_config user_cfg{};
@@ -120,6 +129,9 @@ struct aeros_lbm {
     std::unique_ptr<EngineBase> eng;
     hipStream_t stream = nullptr;
     int device = 0;
+    std::mutex mu;                          // one call (or frame of the frame worker) at a time
+    std::unique_ptr<FrameDriver> frames;    // asynchronous frames, when started
+    ~aeros_lbm();
 };
 
 namespace {
@@ -605,6 +617,41 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         return AEROS_OK;
     }
 
+    int field_device(int id, float* dst) override {
+        const DevBuf* src = nullptr;
+        switch (id) {
+            case AEROS_FIELD_RHO: src = &fld[0]; break;
+            case AEROS_FIELD_UX: src = &fld[1]; break;
+            case AEROS_FIELD_UY: src = &fld[2]; break;
+            case AEROS_FIELD_UZ: src = &fld[3]; break;
+            case AEROS_FIELD_TKE: src = &fld[4]; break;
+            case AEROS_FIELD_STRAIN: src = &fld[5]; break;
+            case AEROS_FIELD_UX_WORLD: src = &world[0]; break;
+            case AEROS_FIELD_UY_WORLD: src = &world[1]; break;
+            case AEROS_FIELD_UZ_WORLD: src = &world[2]; break;
+            case AEROS_FIELD_PRESSURE: src = &world[3]; break;
+            case AEROS_FIELD_VORTICITY: src = vort.p ? &vort : nullptr; break;
+            case AEROS_FIELD_QCRIT: src = qcrit.p ? &qcrit : nullptr; break;
+            case AEROS_FIELD_T:
+            case AEROS_FIELD_BUOYANCY:
+                if (!s.thermal_ready) return set_err(AEROS_ERR_STATE, "thermal module is off");
+                src = id == AEROS_FIELD_T ? &T[cur & 1] : &buoy[cur & 1];
+                break;
+            default: return set_err(AEROS_ERR_ARG, "field %d is not held on the device", id);
+        }
+        if (!src) {   // vorticity / Q never computed: zeros, as read_field gives
+            hipLaunchKernelGGL((k_fill<float>), dim3(grid_for(n)), dim3(256), 0, st, dst, n, 0.0f);
+        } else if constexpr (sizeof(R) == sizeof(float)) {
+            HIP_TRY(hipMemcpyAsync(dst, src->p, (size_t)n * sizeof(float), hipMemcpyDeviceToDevice, st));
+        } else {
+            hipLaunchKernelGGL((k_to_f32<R>), dim3(grid_for(n)), dim3(256), 0, st, src->as<R>(), dst, n);
+        }
+        HIP_TRY(hipGetLastError());
+        return AEROS_OK;
+    }
+
+    int step_count() const override { return cur; }
+
     int read_populations(double* out) override {
         DevBuf tmp;
         int rc = tmp.alloc((size_t)n * Q * sizeof(R));
@@ -832,6 +879,8 @@ int build_handle(aeros_lbm* h) {
 
 }  // namespace
 
+#include "lbm_frames.inc"
+
 // ============================================================================ C ABI
 extern "C" {
 
@@ -879,6 +928,7 @@ aeros_lbm* aeros_lbm_create(const aeros_lbm_config* cfg, int* err) {
 
 int aeros_lbm_destroy(aeros_lbm* h) {
     if (!h) return set_err(AEROS_ERR_ARG, "null handle");
+    h->frames.reset();   // joins the frame worker
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     h->eng.reset();
@@ -889,6 +939,7 @@ int aeros_lbm_destroy(aeros_lbm* h) {
 
 int aeros_lbm_reset(aeros_lbm* h) {
     if (!h) return set_err(AEROS_ERR_ARG, "null handle");
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     h->eng.reset();
@@ -909,6 +960,7 @@ int aeros_lbm_set_mask(aeros_lbm* h, const uint8_t* solid, int32_t nx, int32_t n
     if (!h || !h->eng || !solid) return set_err(AEROS_ERR_ARG, "null argument");
     if (nx != h->s.nx || ny != h->s.ny || nz != h->s.nz)
         return set_err(AEROS_ERR_ARG, "mask %dx%dx%d does not match the lattice %dx%dx%d", nx, ny, nz, h->s.nx, h->s.ny, h->s.nz);
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     Setup& s = h->s;
@@ -925,39 +977,84 @@ int aeros_lbm_set_mask(aeros_lbm* h, const uint8_t* solid, int32_t nx, int32_t n
 int aeros_lbm_step(aeros_lbm* h, int32_t steps) {
     if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
     if (steps < 0) return set_err(AEROS_ERR_ARG, "negative step count");
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     return h->eng->step(steps);
+}
+
+int32_t aeros_lbm_update_steps(float dt, int32_t force_steps, int32_t steps_per_frame) {
+    return update_steps(dt, force_steps, steps_per_frame);
 }
 
 int aeros_lbm_update(aeros_lbm* h, float dt, int32_t force_steps, int32_t steps_per_frame) {
     if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
-    int steps;   // aeros/src/lbm.cpp:1762-1776
-    if (force_steps >= 0) steps = std::min(force_steps, 64);
-    else {
-        steps = steps_per_frame;
-        if (steps <= 0) steps = 1;
-        if (steps > 50) steps = 50;
-        if (dt > 0.02f) steps = std::min(steps * 2, 50);
-    }
-    if (steps <= 0) return AEROS_OK;
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
-    return h->eng->step(steps);
+    return run_update(h, update_steps(dt, force_steps, steps_per_frame));
+}
+
+int aeros_lbm_lattice_dt(aeros_lbm* h, float* out) {
+    if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    *out = lattice_dt_world(h->s);
+    return AEROS_OK;
+}
+
+int aeros_lbm_frames_start(aeros_lbm* h, uint32_t field_mask) {
+    if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
+    if (h->frames) return set_err(AEROS_ERR_STATE, "frames are already running");
+    if (field_mask & ~kFrameFields) return set_err(AEROS_ERR_ARG, "field mask 0x%x names a field not held on the device", field_mask);
+    if ((field_mask & ((1u << AEROS_FIELD_T) | (1u << AEROS_FIELD_BUOYANCY))) && !h->s.thermal_ready)
+        return set_err(AEROS_ERR_STATE, "thermal module is off");
+    (void)hipSetDevice(h->device);
+    std::unique_ptr<FrameDriver> d(new (std::nothrow) FrameDriver(h));
+    if (!d) return set_err(AEROS_ERR_NOMEM, "frame driver");
+    const int rc = d->start(field_mask);
+    if (rc) return rc;
+    h->frames = std::move(d);
+    return AEROS_OK;
+}
+
+int aeros_lbm_frame_submit(aeros_lbm* h, float dt, int32_t force_steps, int32_t steps_per_frame) {
+    if (!h || !h->frames) return set_err(AEROS_ERR_STATE, "frames are not running");
+    return h->frames->submit(FrameDriver::Req{dt, force_steps, steps_per_frame});
+}
+
+int aeros_lbm_frame_latest(aeros_lbm* h, aeros_frame* out) {
+    if (!h || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    if (!h->frames) return set_err(AEROS_ERR_STATE, "frames are not running");
+    return h->frames->latest(out);
+}
+
+int aeros_lbm_frames_wait(aeros_lbm* h) {
+    if (!h || !h->frames) return set_err(AEROS_ERR_STATE, "frames are not running");
+    return h->frames->wait();
+}
+
+int aeros_lbm_frames_stop(aeros_lbm* h) {
+    if (!h) return set_err(AEROS_ERR_ARG, "null handle");
+    if (!h->frames) return AEROS_OK;
+    const int rc = h->frames->wait();
+    h->frames.reset();
+    return rc;
 }
 
 int aeros_lbm_read_field(aeros_lbm* h, int32_t field, float* out) {
     if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     return h->eng->read_field(field, nullptr, out);
 }
 
 int aeros_lbm_read_field_f64(aeros_lbm* h, int32_t field, double* out) {
     if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     return h->eng->read_field(field, out, nullptr);
 }
 
 int aeros_lbm_read(aeros_lbm* h, uint32_t fields, float* rho, float* ux, float* uy, float* uz, float* T) {
     if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     struct { uint32_t bit; int id; float* p; } v[5] = {{AEROS_READ_RHO, AEROS_FIELD_RHO, rho}, {AEROS_READ_UX, AEROS_FIELD_UX, ux},
                                                      {AEROS_READ_UY, AEROS_FIELD_UY, uy}, {AEROS_READ_UZ, AEROS_FIELD_UZ, uz},
@@ -972,18 +1069,21 @@ int aeros_lbm_read(aeros_lbm* h, uint32_t fields, float* rho, float* ux, float* 
 
 int aeros_lbm_get_scalars(aeros_lbm* h, aeros_lbm_scalars* out) {
     if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock_(h->mu);
     h->eng->scalars(out);
     return AEROS_OK;
 }
 
 int aeros_lbm_read_populations(aeros_lbm* h, double* out) {
     if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     return h->eng->read_populations(out);
 }
 
 int aeros_lbm_cv_force(aeros_lbm* h, float hx, float hy, float hz, double out[3]) {
     if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     return h->eng->cv_force(hx, hy, hz, out);
 }
@@ -991,18 +1091,21 @@ int aeros_lbm_cv_force(aeros_lbm* h, float hx, float hy, float hz, double out[3]
 int aeros_lbm_projected_area(aeros_lbm* h, int32_t axis, float* out) {
     if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
     if (axis < 0 || axis > 2) return set_err(AEROS_ERR_ARG, "axis %d", axis);
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     return h->eng->projected_area(axis, out);
 }
 
 int aeros_lbm_checksums(aeros_lbm* h, double* rho_sum, double* u_sum) {
     if (!h || !h->eng || !rho_sum || !u_sum) return set_err(AEROS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     return h->eng->checksums(rho_sum, u_sum);
 }
 
 int aeros_lbm_thermal_upload(aeros_lbm* h, const float* T) {
     if (!h || !h->eng || !T) return set_err(AEROS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     return h->eng->thermal_upload(T);
 }
@@ -1012,6 +1115,7 @@ int aeros_lbm_thermal_reset(aeros_lbm* h) {
     if (!h->s.thermal_ready) return set_err(AEROS_ERR_STATE, "thermal module not initialised");
     std::vector<float> T;
     thermal_initial_field(h->s, T);
+    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     return h->eng->thermal_upload(T.data());
 }
@@ -1076,18 +1180,21 @@ int aeros_voxelize_gpu(const float* tris, int64_t ntris, int32_t nx, int32_t ny,
 
 int aeros_lbm_vorticity_q(aeros_lbm* h) {
     if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
+    std::lock_guard<std::mutex> lock_(h->mu);
     HIP_TRY(hipSetDevice(h->device));
     return h->eng->vorticity_q();
 }
 
 int aeros_lbm_sample(aeros_lbm* h, int32_t field, const float* pos, int64_t npos, float* out, uint8_t* valid) {
     if (!h || !h->eng || npos < 0 || (npos > 0 && (!pos || !out))) return set_err(AEROS_ERR_ARG, "bad sample arguments");
+    std::lock_guard<std::mutex> lock_(h->mu);
     HIP_TRY(hipSetDevice(h->device));
     return h->eng->sample(field, pos, npos, out, valid);
 }
 
 int aeros_lbm_sample_velocity(aeros_lbm* h, int32_t world, const float* pos, int64_t npos, float* out) {
     if (!h || !h->eng || npos < 0 || (npos > 0 && (!pos || !out))) return set_err(AEROS_ERR_ARG, "bad sample arguments");
+    std::lock_guard<std::mutex> lock_(h->mu);
     HIP_TRY(hipSetDevice(h->device));
     return h->eng->sample_velocity(world, pos, npos, out);
 }
@@ -1257,6 +1364,7 @@ int aeros_probe(aeros_backend_info* out) {
 
 int aeros_lbm_kernel_timing(aeros_lbm* h, int32_t enable, double* total_ms, int64_t* launches) {
     if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
+    std::lock_guard<std::mutex> lock_(h->mu);
     EngineBase* e = h->eng.get();
     if (enable < 0) { e->timed_ms = 0.0; e->timed_launches = 0; e->timing = false; }
     else e->timing = enable != 0;

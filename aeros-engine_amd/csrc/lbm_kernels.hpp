@@ -1657,6 +1657,12 @@ __global__ __launch_bounds__(256) void k_voxelize(const VoxNode* nodes, const Vo
     }
 }
 
+// A field of R values as float (the asynchronous frames publish floats).
+template <class R> __global__ void k_to_f32(const R* in, float* out, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
 // Fill a typed buffer with a value (graph-capturable, unlike hipMemset).
 template <class T> __global__ void k_fill(T* p, int64_t n, T v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;

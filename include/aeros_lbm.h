@@ -244,6 +244,69 @@ int aeros_lbm_step(aeros_lbm* h, int32_t steps);
  * forceSteps >= 0 is clamped to 64; -1 uses stepsPerFrame with the frame-time
  * rule.  steps_per_frame is LBMParams::stepsPerFrame. */
 int aeros_lbm_update(aeros_lbm* h, float dt, int32_t force_steps, int32_t steps_per_frame);
+/* The lattice steps updateLBM runs for these arguments (host rule only). */
+int32_t aeros_lbm_update_steps(float dt, int32_t force_steps, int32_t steps_per_frame);
+/* lbmLatticeDtWorld (aeros/src/lbm.h:171, aeros/src/lbm.cpp:1717-1732):
+ * physical seconds one lattice step represents, cell size / (flowSpeed / |u_in|). */
+int aeros_lbm_lattice_dt(aeros_lbm* h, float* out);
+
+/* ------------------------------------------------- host stepping driver */
+
+/* Fixed-tick physics clock (PhysClock, aeros/src/phys_clock.h:48-250):
+ * settings first (as the reference's defaults: fixed, 240 Hz, 64 ticks per
+ * frame, 8 particle sub-steps, frames cut at 0.25 s), then state and the
+ * last frame's measurements. */
+typedef struct aeros_phys_clock {
+    int32_t fixed;                /* 0 = one tick per frame, the solver's own steps per frame */
+    float   hz;
+    int32_t max_substeps;
+    int32_t particle_substeps;
+    float   max_frame_dt;
+    float   accumulator;          /* real seconds not yet ticked */
+    float   lattice_accum;        /* fraction of a lattice step carried over */
+    double  sim_time, real_time, dropped_time;
+    int64_t total_ticks, total_lattice_steps;
+    int32_t ticks, lattice_steps, part_steps;
+    float   tick_dt, achieved_hz, time_scale, wall_scale, display_ratio;
+    int32_t saturated;            /* the simulation fell behind real time this frame */
+} aeros_phys_clock;
+
+void aeros_clock_defaults(aeros_phys_clock* c);
+void aeros_clock_reset(aeros_phys_clock* c);        /* state only: the settings stay */
+void aeros_clock_sanitize(aeros_phys_clock* c);     /* clamps settings and accumulators */
+/* One rendered frame of frame_dt seconds: returns the physics ticks and puts
+ * the lattice steps (lattice_dt seconds each; <= 0 or NaN: none) in
+ * *lbm_steps (-1 when fixed = 0: the solver chooses). */
+int32_t aeros_clock_advance(aeros_phys_clock* c, float frame_dt, float lattice_dt, float fps, int32_t* lbm_steps);
+float aeros_clock_particle_dt(const aeros_phys_clock* c);
+float aeros_clock_frame_sim_dt(const aeros_phys_clock* c);
+
+/* Asynchronous frames (no counterpart in the reference, whose updateLBM
+ * blocks the frame loop).  After aeros_lbm_frames_start a worker thread runs
+ * each submitted frame as aeros_lbm_update does (the same step rule, then
+ * computeLBMVorticityAndQ whenever the step count is a multiple of 30), copies
+ * the fields of field_mask (bits 1 << AEROS_FIELD_*; device fields only, not
+ * SOLID/GROUND/WALLDIST) into a device staging buffer and from there, on a
+ * copy stream overlapped with the next frame's steps, into pinned host
+ * memory.  Three host slots rotate: the caller holds one, one holds the
+ * newest finished frame, one is being written, so neither side waits for the
+ * other.  Every other call on the handle waits for the frame in progress. */
+typedef struct aeros_frame {
+    int64_t frame;                /* 1, 2, ... in submission order; 0 = none finished yet */
+    int32_t steps;                /* lattice steps that frame ran */
+    aeros_lbm_scalars scalars;    /* after that frame */
+    const float* field[AEROS_FIELD_COUNT];   /* Nx*Ny*Nz floats each, NULL if not in the mask */
+} aeros_frame;
+int aeros_lbm_frames_start(aeros_lbm* h, uint32_t field_mask);
+/* Queues one frame (at most two wait; a third submission blocks until the
+ * worker takes one).  Returns the first error a finished frame met, if any. */
+int aeros_lbm_frame_submit(aeros_lbm* h, float dt, int32_t force_steps, int32_t steps_per_frame);
+/* Never blocks: the newest finished frame, whose buffers stay valid until the
+ * next call of aeros_lbm_frame_latest or aeros_lbm_frames_stop. */
+int aeros_lbm_frame_latest(aeros_lbm* h, aeros_frame* out);
+/* Blocks until every submitted frame has finished and been copied. */
+int aeros_lbm_frames_wait(aeros_lbm* h);
+int aeros_lbm_frames_stop(aeros_lbm* h);           /* waits, then frees the buffers */
 
 /* ---------------------------------------------------------------- readback */
 

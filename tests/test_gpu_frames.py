@@ -1,0 +1,97 @@
+"""GPU: the host stepping driver's asynchronous frames against the same
+updateLBM calls made synchronously on a twin lattice (bit for bit), the
+vorticity/Q pass every 30 steps, non-blocking reads and error paths."""
+import time
+
+import numpy as np
+import pytest
+
+import aeros_lbm as A
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("rho", "ux", "uy", "uz", "pressure", "vorticity", "qcrit")
+PLAN = [(0.016, 5, 3), (0.016, -1, 3), (0.03, -1, 4), (0.016, 0, 3), (0.016, 11, 3), (0.016, -1, 5)]
+
+
+def twins():
+    cfg = W.bench_sphere(16)
+    print("frames on bench_sphere(16)", flush=True)
+    return A.Lattice(A.default_config(**cfg)), A.Lattice(A.default_config(**cfg))
+
+
+def test_frames_match_synchronous_updates():
+    F, S = twins()
+    F.frames_start(FIELDS)
+    first = F.frame_latest()
+    assert first["frame"] == 0 and first["fields"] == {}      # nothing finished yet: no wait
+    for dt, force, spf in PLAN:
+        F.frame_submit(dt, force, spf)
+    F.frames_wait()
+    for dt, force, spf in PLAN:
+        S.update(dt, force, spf)
+    fr = F.frame_latest()
+    assert fr["frame"] == len(PLAN)
+    assert fr["steps"] == A.update_steps(*PLAN[-1])
+    assert fr["scalars"]["step"] == S.scalars()["step"] == sum(A.update_steps(*p) for p in PLAN)
+    for name in FIELDS:
+        np.testing.assert_array_equal(fr["fields"][name], S.field(name), err_msg=name)
+    F.frames_stop()
+
+
+def test_vorticity_every_30_steps():
+    F, S = twins()
+    F.frames_start(("vorticity", "qcrit", "ux"))
+    F.frame_submit(0.016, 30, 3)          # step 30: the pass runs
+    F.frame_submit(0.016, 7, 3)           # step 37: it does not
+    F.frames_wait()
+    fr = F.frame_latest()
+    S.step(30)
+    S.vorticity_q()
+    vort30 = S.field("vorticity")
+    S.step(7)
+    np.testing.assert_array_equal(fr["fields"]["vorticity"], vort30)
+    np.testing.assert_array_equal(fr["fields"]["ux"], S.field("ux"))
+    assert float(np.abs(vort30).max()) > 0
+    F.frames_stop()
+
+
+def test_reads_while_frames_run_are_monotonic_and_never_block():
+    F, _ = twins()
+    F.frames_start(("rho",))
+    seen, worst = [], 0.0
+    for _ in range(40):
+        F.frame_submit(0.016, 20, 3)
+        t = time.perf_counter()
+        fr = F.frame_latest(copy=False)
+        worst = max(worst, time.perf_counter() - t)
+        seen.append(fr["frame"])
+        F.scalars()                        # other calls wait for the frame in progress
+    F.frames_wait()
+    seen.append(F.frame_latest()["frame"])
+    assert seen == sorted(seen) and seen[-1] == 40
+    assert worst < 0.05, worst
+    F.frames_stop()
+    F.step(3)                              # the handle works as before after stop
+
+
+def test_frame_errors():
+    F, _ = twins()
+    with pytest.raises(A.AerosError):
+        F.frame_submit(0.016, 1, 3)        # not started
+    with pytest.raises(A.AerosError):
+        F.frames_start(("solid",))         # host-side field
+    with pytest.raises(A.AerosError):
+        F.frames_start(("T",))             # thermal module off
+    F.frames_start(("rho",))
+    with pytest.raises(A.AerosError):
+        F.frames_start(("rho",))           # already running
+    F.frames_stop()
+
+
+def test_lattice_dt_world():
+    F, _ = twins()
+    # cell size / (flowSpeed / |u_in|): bench sphere, 20 m/s over U0 0.08
+    dx = F.cell_size[0]
+    assert F.lattice_dt() == pytest.approx(dx / (20.0 / 0.08), rel=1e-6)
