@@ -90,9 +90,7 @@ void fill_config(aeros_lbm_config& c) {
 }
 
 // The scalar globals of aeros/src/lbm.h:77-125 and aeros/src/thermal_lbm.h:46-58.
-void pull_scalars() {
-    aeros_lbm_scalars s;
-    if (aeros_lbm_get_scalars(g_hip, &s) != AEROS_OK) return;
+void apply_scalars(const aeros_lbm_scalars& s) {
     lbmCurrentStep = s.step;
     lbmConverged = s.converged != 0;
     lbmAvgRho = s.avg_rho;
@@ -125,6 +123,11 @@ void pull_scalars() {
     thermalGravityLb = s.thermal_gravity;
     thermalRayleigh = s.thermal_rayleigh;
     thermalPrandtl = s.thermal_prandtl;
+}
+
+void pull_scalars() {
+    aeros_lbm_scalars s;
+    if (aeros_lbm_get_scalars(g_hip, &s) == AEROS_OK) apply_scalars(s);
 }
 
 void pull_field(int32_t id, std::vector<float>& dst, size_t n) {
@@ -171,10 +174,40 @@ void pull_geometry() {
         for (size_t i = 0; i < n; ++i) lbmIsGround[i] = tmp[i] != 0.0f;
 }
 
+// Asynchronous frames: whether they run, and the newest frame in the globals.
+bool g_frames = false;
+int64_t g_frame_seen = 0;
+
+// The fields pull_fields copies, as they arrive with a finished frame.
+struct FrameField {
+    int32_t id;
+    std::vector<float>* dst;
+};
+std::vector<FrameField> frame_fields() {
+    std::vector<FrameField> v = {{AEROS_FIELD_RHO, &lbmRho}, {AEROS_FIELD_UX, &lbmUx}, {AEROS_FIELD_UY, &lbmUy},
+                                 {AEROS_FIELD_UZ, &lbmUz}, {AEROS_FIELD_UX_WORLD, &lbmUxWorld},
+                                 {AEROS_FIELD_UY_WORLD, &lbmUyWorld}, {AEROS_FIELD_UZ_WORLD, &lbmUzWorld},
+                                 {AEROS_FIELD_PRESSURE, &lbmPressure}, {AEROS_FIELD_TKE, &lbmTKEField},
+                                 {AEROS_FIELD_STRAIN, &lbmStrainMag}, {AEROS_FIELD_VORTICITY, &lbmVorticityMag},
+                                 {AEROS_FIELD_QCRIT, &lbmQCriterion}};
+    if (thermalReady) {
+        v.push_back({AEROS_FIELD_T, &thermalT});
+        v.push_back({AEROS_FIELD_BUOYANCY, &thermalBuoyancyY});
+    }
+    return v;
+}
+
+void stop_frames() {
+    if (g_hip && g_frames) aeros_lbm_frames_stop(g_hip);
+    g_frames = false;
+    g_frame_seen = 0;
+}
+
 }  // namespace
 
 // initLBM (aeros/src/lbm.cpp:405-886) on the device.
 bool aerosHipInitLBM() {
+    stop_frames();
     if (g_hip) { aeros_lbm_destroy(g_hip); g_hip = nullptr; }
     aeros_lbm_config c;
     fill_config(c);
@@ -191,6 +224,7 @@ bool aerosHipInitLBM() {
 // resetLBM (aeros/src/lbm.cpp:888-895).
 bool aerosHipResetLBM() {
     if (!g_hip) return aerosHipInitLBM();
+    stop_frames();
     if (aeros_lbm_reset(g_hip) != AEROS_OK) { report("aeros_lbm_reset"); return false; }
     pull_scalars();
     pull_fields();
@@ -199,6 +233,7 @@ bool aerosHipResetLBM() {
 
 // shutdownLBM (aeros/src/lbm.cpp:897-905).
 void aerosHipShutdownLBM() {
+    stop_frames();
     if (g_hip) aeros_lbm_destroy(g_hip);
     g_hip = nullptr;
     lbmInitialized = false;
@@ -223,6 +258,40 @@ bool aerosHipUpdateLBM(float dt, int forceSteps) {
     pull_scalars();
     pull_fields();
     return true;
+}
+
+// updateLBM without blocking the frame loop: the frame is queued on the
+// library's frame worker (the same steps and 30-step vorticity pass), and the
+// globals take the newest finished frame, one or two frames behind, from
+// pinned host memory.  Any synchronous call (stepLBMCPU, a reset) still
+// works and waits for the frame in progress.
+bool aerosHipUpdateLBMAsync(float dt, int forceSteps) {
+    if (!g_hip) return false;
+    if (!g_frames) {
+        uint32_t mask = 0;
+        for (const FrameField& f : frame_fields()) mask |= 1u << f.id;
+        if (aeros_lbm_frames_start(g_hip, mask) != AEROS_OK) { report("aeros_lbm_frames_start"); return false; }
+        g_frames = true;
+    }
+    if (aeros_lbm_frame_submit(g_hip, dt, forceSteps, lbmParams.stepsPerFrame) != AEROS_OK) {
+        report("aeros_lbm_frame_submit");
+        return false;
+    }
+    aeros_frame fr;
+    if (aeros_lbm_frame_latest(g_hip, &fr) != AEROS_OK) { report("aeros_lbm_frame_latest"); return false; }
+    if (fr.frame > g_frame_seen) {
+        const size_t n = (size_t)lbmNx * (size_t)lbmNy * (size_t)lbmNz;
+        for (const FrameField& f : frame_fields())
+            if (fr.field[f.id]) f.dst->assign(fr.field[f.id], fr.field[f.id] + n);
+        apply_scalars(fr.scalars);
+        g_frame_seen = fr.frame;
+    }
+    return true;
+}
+
+// lbmLatticeDtWorld (aeros/src/lbm.cpp:1717-1732).
+bool aerosHipLatticeDtWorld(float* out) {
+    return g_hip && aeros_lbm_lattice_dt(g_hip, out) == AEROS_OK;
 }
 
 // lbmControlVolumeForce (aeros/src/lbm.cpp:2183-2232): the double sums, cast like the reference's return.
