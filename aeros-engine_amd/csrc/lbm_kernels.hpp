@@ -81,10 +81,22 @@ template <> struct Pop<double, double> {
     __device__ static double ld(const double* p, uint32_t i, double) { return at(p, i); }
     __device__ static void st(double* p, uint32_t i, double v, double) { st_nt(at(p, i), v); }
 };
+// h + w for a half h, and the half nearest to v - w: each one mixed-precision
+// multiply-add (h * 1 + w in FP32; v * 1 - w rounded once to half), where
+// the compiler would otherwise spend a conversion and an add on each.
+__device__ inline float half_plus(_Float16 h, float w) {
+    float r;
+    asm("v_fma_mix_f32 %0, %1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(w));
+    return r;
+}
+__device__ inline _Float16 half_minus(float v, float w) {
+    _Float16 r;
+    asm("v_fma_mixlo_f16 %0, %1, 1.0, -%2" : "=v"(r) : "v"(v), "v"(w));
+    return r;
+}
 template <> struct Pop<_Float16, float> {   // deviation from w_q, IEEE half, round to nearest even
-    // (float)h + w with a single rounding; the conversion is exact
-    __device__ static float ld(const _Float16* p, uint32_t i, float w) { return __builtin_fmaf((float)at(p, i), 1.0f, w); }
-    __device__ static void st(_Float16* p, uint32_t i, float v, float w) { st_nt(at(p, i), (_Float16)(v - w)); }
+    __device__ static float ld(const _Float16* p, uint32_t i, float w) { return half_plus(at(p, i), w); }
+    __device__ static void st(_Float16* p, uint32_t i, float v, float w) { st_nt(at(p, i), half_minus(v, w)); }
 };
 __device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 __device__ inline uint16_t f_to_bf16(float x) {   // round to nearest even

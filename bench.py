@@ -59,11 +59,15 @@ def run_gpu(A, cfg, steps, warmup, label, T=None):
         L.thermal_upload(T)
     log(f"[{label}] lattice {L.dims} ready in {time.time() - t0:.1f} s")
     L.step(warmup)
-    L.kernel_timing(-1)
-    L.kernel_timing(1)
+    # throughput: the K steps alone (an event pair around every launch would
+    # hold back the next dispatch and lower the rate by 4-20 % on these sizes)
     t1 = time.perf_counter()
     L.step(steps)
     dt = time.perf_counter() - t1
+    # step-kernel duration: HIP events around each launch on the solver's stream
+    L.kernel_timing(-1)
+    L.kernel_timing(1)
+    L.step(min(steps, 300))
     kms, launches = L.kernel_timing(0)
     sc = L.scalars()
     cells = L.ncells
@@ -140,6 +144,36 @@ def solve_alone(A, W, cfg, round_steps, tol, max_steps):
         prev = cd
     L.close()
     return cd, steps
+
+
+def host_driver_leg(A, cfg, frames=40, steps=10, fields=("rho", "ux", "uy", "uz")):
+    """The host stepping driver on the headline case: frames of `steps` lattice steps
+    whose fields reach host memory.  Synchronous: update() then a read of each field.
+    Asynchronous: the frame worker stages the fields on the device and copies them to a
+    pinned slot while the next frame steps; the caller takes the newest finished frame."""
+    L = A.Lattice(A.default_config(**cfg))
+    L.step(20)
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        L.update(0.016, steps, 3)
+        for f in fields:
+            L.field(f)
+    t_sync = time.perf_counter() - t0
+    L.frames_start(fields)
+    t0 = time.perf_counter()
+    for _ in range(frames):
+        L.frame_submit(0.016, steps, 3)
+        L.frame_latest(copy=False)
+    L.frames_wait()
+    t_async = time.perf_counter() - t0
+    last = L.frame_latest(copy=False)["frame"]
+    L.frames_stop()
+    L.close()
+    res = {"frames": frames, "steps_per_frame": steps, "fields": list(fields),
+           "sync_frames_per_s": round(frames / t_sync, 1), "async_frames_per_s": round(frames / t_async, 1),
+           "async_over_sync": round(t_sync / t_async, 2), "last_frame": last}
+    log(f"[host driver] {res}")
+    return res
 
 
 def served_leg(A, W, n=256, base_k=32, round_steps=200, tol=1e-4, max_steps=4000, seed=5150, load=4.0):
@@ -343,6 +377,7 @@ def main():
                             "c1": "96x48x48 FP64 BGK sphere D=12, U0 0.05, zero-gradient outflow, "
                                   "free-slip side walls",
                             "c3": "128x64x128 coupled D3Q19+D3Q7 FP32, Rayleigh-Benard Ra 2500 Pr 0.71"}
+        sec["host_driver"] = host_driver_leg(A, c2)
         out["secondary"] = sec
     if not args.no_served:
         out["served"] = served_leg(A, W)
