@@ -290,3 +290,15 @@ def c5_requests(n=256, seed=5150, **kw) -> list[tuple[dict, dict]]:
     """The served-mode stream: n seeded requests (PCG64(seed))."""
     rng = np.random.Generator(np.random.PCG64(seed))
     return [c5_request(rng, **kw) for _ in range(n)]
+
+
+def reference_field_cases() -> dict:
+    """The small cases whose per-cell fields the reference solver dumped
+    (tests/golden/ref_fields.json): name -> (config overrides, steps)."""
+    return {
+        "bench12_les": (bench_sphere(12, les=True), 90),
+        "bench12_les_thermal": (bench_sphere(12, les=True, thermal=True), 90),
+        "bench16": (bench_sphere(16, les=False), 150),
+        "rb_walls": (thermal_box(32, 18, 8, walls_y=True, rayleigh=2500.0, fixed_wall_temps=True,
+                                 perturbation=0.02), 200),
+    }

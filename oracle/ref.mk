@@ -24,7 +24,7 @@ CORE    := $(SRC)/lbm.cpp $(SRC)/thermal_lbm.cpp $(SRC)/flow_field.cpp $(SRC)/vo
            $(SRC)/atmosphere.cpp $(SRC)/globals.cpp $(SRC)/debug_log.cpp
 COMMON  := $(SRC)/globals.cpp $(SRC)/debug_log.cpp
 
-BINS := bench bench_fma lbm physref thermal validate voxel
+BINS := bench bench_fma lbm physref thermal validate voxel ref_fields
 .PHONY: all $(BINS)
 all: $(BINS)
 
@@ -51,3 +51,10 @@ validate: | $(OUT)
 
 voxel: | $(OUT)
 	$(CXX) $(FLAGS) $(T)/voxel_headless.cpp $(SRC)/voxel_grid.cpp $(COMMON) -o $(OUT)/voxel -pthread
+
+# Per-cell field dumps of the reference solver (oracle/ref_fields.cpp, test
+# infrastructure); the link stubs come from the reference bench's own
+# translation unit with its main renamed.
+ref_fields: | $(OUT)
+	$(CXX) $(FLAGS) -Dmain=aeros_bench_main_unused -c $(T)/bench_lbm.cpp -o $(OUT)/bench_stubs.o
+	$(CXX) $(FLAGS) $(HERE)ref_fields.cpp $(OUT)/bench_stubs.o $(CORE) -o $(OUT)/ref_fields -pthread

@@ -1,0 +1,134 @@
+// TEST INFRASTRUCTURE ONLY.  Runs the reference solver (compiled from its
+// own sources under the reference tree by oracle/ref.mk) on small cases and
+// writes its per-cell fields as raw little-endian float32 files into
+// oracle/_ref/fields/, from which tests/golden/make_golden.py records SHA-256
+// digests: the per-cell pin of the oracle's FP32 arithmetic (convective
+// outlet, LES with Van Driest damping near walls, thermal coupling,
+// Dirichlet wall rows) that the summed checksums only pin in aggregate.
+//
+// The link-time stubs the reference's solver files need (its GLFW clock, the
+// CUDA and renderer hooks) come from the reference's own bench translation
+// unit, built with its main renamed; this file adds none.
+//
+//   oracle/_ref/ref_fields <case> <out-dir>
+//   cases: bench12_les  bench12_les_thermal  bench16  rb_walls
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "atmosphere.h"
+#include "globals.h"
+#include "lbm.h"
+#include "thermal_lbm.h"
+
+namespace {
+
+// The reference bench's sphere (aeros/tests/bench_lbm.cpp:43-79): radius 0.35
+// in [-1, 1]^3 on N^3 voxels, 25 % padding, tau 0.6, U0 0.08, convective outlet.
+void sphere_case(int n, bool les, bool thermal) {
+    g_voxNx = g_voxNy = g_voxNz = n;
+    g_voxMinX = g_voxMinY = g_voxMinZ = -1.0f;
+    g_voxMaxX = g_voxMaxY = g_voxMaxZ = 1.0f;
+    g_voxelData.assign((size_t)n * n * n, 0);
+    g_distanceField.assign((size_t)n * n * n, 0.0f);
+    const float radius = 0.35f, h = 2.0f / (float)n;
+    for (int z = 0; z < n; ++z)
+        for (int y = 0; y < n; ++y)
+            for (int x = 0; x < n; ++x) {
+                const float px = -1.0f + (x + 0.5f) * h, py = -1.0f + (y + 0.5f) * h, pz = -1.0f + (z + 0.5f) * h;
+                const float d = std::sqrt(px * px + py * py + pz * pz) - radius;
+                g_voxelData[((size_t)z * n + y) * n + x] = d <= 0.0f ? 1 : 0;
+                g_distanceField[((size_t)z * n + y) * n + x] = d / h;
+            }
+    minBB = glm::vec3(-radius);
+    maxBB = glm::vec3(radius);
+    center = glm::vec3(0.0f);
+    maxDim = 2.0f * radius;
+    voxelResolution = n;
+    flowSpeed = 20.0f; flowAzimuth = 0.0f; flowElevation = 0.0f;
+    altitude = 0.0f; timeScale = 1.0f; aeroMachEffects = false;
+    lbmParams.enabled = true; lbmParams.useGPU = false;
+    lbmParams.U0 = 0.08f; lbmParams.stepsPerFrame = 1; lbmParams.tau = 0.6f;
+    lbmParams.useTurbulence = les; lbmParams.useGround = false; lbmParams.inletTurbulence = 0.0f;
+    lbmParams.domainPadding = 0.25f; lbmParams.autoViscosity = false;
+    lbmParams.useConvectiveOutlet = true; lbmParams.periodicX = false;
+    lbmParams.bodyForceX = lbmParams.bodyForceY = lbmParams.bodyForceZ = 0.0f;
+    thermalParams.enabled = thermal;
+    thermalParams.rayleigh = 2500.0f;
+    thermalParams.prandtl = 1.0f;
+}
+
+// The reference's Rayleigh-Benard box (aeros/tests/thermal_headless.cpp:54-99,
+// 295-304): 32x18x8, solid rows y = 0 and y = 17, periodic x and z, tau 0.65,
+// fixed wall temperatures 1 / 0, Ra 2500, Pr 1, perturbation 0.02.
+void rb_case() {
+    const int nx = 32, ny = 18, nz = 8;
+    g_voxNx = nx; g_voxNy = ny; g_voxNz = nz;
+    g_voxMinX = g_voxMinY = g_voxMinZ = -1.0f;
+    g_voxMaxX = g_voxMaxY = g_voxMaxZ = 1.0f;
+    g_voxelData.assign((size_t)nx * ny * nz, 0);
+    g_distanceField.assign((size_t)nx * ny * nz, 0.0f);
+    for (int z = 0; z < nz; ++z)
+        for (int x = 0; x < nx; ++x) {
+            g_voxelData[((size_t)z * ny + 0) * nx + x] = 1;
+            g_voxelData[((size_t)z * ny + ny - 1) * nx + x] = 1;
+        }
+    minBB = glm::vec3(-1.0f);
+    maxBB = glm::vec3(1.0f);
+    center = glm::vec3(0.0f);
+    maxDim = 2.0f;
+    voxelResolution = nx;
+    flowSpeed = 1.0f; flowAzimuth = 0.0f; flowElevation = 0.0f;
+    altitude = 0.0f; timeScale = 1.0f; aeroMachEffects = false;
+    lbmParams.enabled = true; lbmParams.useGPU = false;
+    lbmParams.U0 = 0.0f; lbmParams.stepsPerFrame = 1; lbmParams.tau = 0.65f;
+    lbmParams.useTurbulence = false; lbmParams.useGround = false; lbmParams.inletTurbulence = 0.0f;
+    lbmParams.domainPadding = 0.0f; lbmParams.autoViscosity = false;
+    lbmParams.useConvectiveOutlet = false; lbmParams.periodicX = true; lbmParams.periodicZ = true;
+    lbmParams.bodyForceX = lbmParams.bodyForceY = lbmParams.bodyForceZ = 0.0f;
+    thermalParams.enabled = true;
+    thermalParams.rayleigh = 2500.0f;
+    thermalParams.prandtl = 1.0f;
+    thermalParams.fixedWallTemps = true;
+    thermalParams.hotTemp = 1.0f;
+    thermalParams.coldTemp = 0.0f;
+    thermalParams.perturbation = 0.02f;
+}
+
+bool dump(const std::string& dir, const char* name, const std::vector<float>& v) {
+    const std::string path = dir + "/" + name + ".f32";
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(v.data(), sizeof(float), v.size(), f) == v.size();
+    return std::fclose(f) == 0 && ok;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 3) {
+        std::fprintf(stderr, "usage: %s <case> <out-dir>\n", argv[0]);
+        return 2;
+    }
+    const std::string which = argv[1], dir = argv[2];
+    int steps = 0;
+    if (which == "bench12_les") { sphere_case(12, true, false); steps = 90; }
+    else if (which == "bench12_les_thermal") { sphere_case(12, true, true); steps = 90; }
+    else if (which == "bench16") { sphere_case(16, false, false); steps = 150; }
+    else if (which == "rb_walls") { rb_case(); steps = 200; }
+    else { std::fprintf(stderr, "unknown case %s\n", which.c_str()); return 2; }
+    shutdownLBM();
+    initLBM();
+    if (!lbmInitialized) { std::fprintf(stderr, "init failed\n"); return 1; }
+    for (int i = 0; i < steps; ++i) stepLBMCPU(1);
+    bool ok = dump(dir, "rho", lbmRho) && dump(dir, "ux", lbmUx) && dump(dir, "uy", lbmUy) && dump(dir, "uz", lbmUz) &&
+              dump(dir, "tke", lbmTKEField) && dump(dir, "strain", lbmStrainMag) && dump(dir, "ux_world", lbmUxWorld) &&
+              dump(dir, "pressure", lbmPressure);
+    if (ok && thermalParams.enabled) ok = dump(dir, "T", thermalT) && dump(dir, "buoyancy", thermalBuoyancyY);
+    std::printf("%s %dx%dx%d steps %d\n", which.c_str(), lbmNx, lbmNy, lbmNz, steps);
+    shutdownLBM();
+    return ok ? 0 : 1;
+}

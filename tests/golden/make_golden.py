@@ -144,14 +144,41 @@ def lbm_suite() -> dict:
     return d
 
 
+# Per-cell field dumps of oracle/_ref/ref_fields (the reference solver run on
+# small cases): recorded as SHA-256 digests of the raw float32 bytes, plus a
+# few values for a readable failure.
+FIELD_CASES = ["bench12_les", "bench12_les_thermal", "bench16", "rb_walls"]
+
+
+def fields() -> dict:
+    import hashlib
+    import numpy as np
+    out = {"cases": {}}
+    for case in FIELD_CASES:
+        d = ROOT / "oracle" / "_ref" / "fields" / case
+        d.mkdir(parents=True, exist_ok=True)
+        text = run("ref_fields", case, str(d))
+        grid, steps = re.search(r"(\d+)x(\d+)x(\d+) steps (\d+)", text).groups()[:3], int(re.search(r"steps (\d+)", text).group(1))
+        rec = {"grid": [int(g) for g in grid], "steps": steps, "digests": {}, "samples": {}}
+        for f in sorted(d.glob("*.f32")):
+            raw = f.read_bytes()
+            v = np.frombuffer(raw, dtype="<f4")
+            rec["digests"][f.stem] = hashlib.sha256(raw).hexdigest()
+            idx = [0, v.size // 3, v.size // 2, v.size - 1]
+            rec["samples"][f.stem] = {str(i): float(v[i]) for i in idx}
+        out["cases"][case] = rec
+    return out
+
+
 def main() -> None:
     if not (REF / "bench").exists():
         raise SystemExit("oracle/_ref is not built: make -f oracle/ref.mk -j8")
-    which = sys.argv[1:] or ["bench", "physref", "thermal", "validate", "lbm"]
-    funcs = {"bench": bench, "physref": physref, "thermal": thermal, "validate": validate, "lbm": lbm_suite}
+    which = sys.argv[1:] or ["bench", "physref", "thermal", "validate", "lbm", "fields"]
+    funcs = {"bench": bench, "physref": physref, "thermal": thermal, "validate": validate, "lbm": lbm_suite,
+             "fields": fields}
     for name in which:
         data = funcs[name]()
-        data["_source"] = f"oracle/_ref/{name if name != 'lbm' else 'lbm'} built by oracle/ref.mk"
+        data["_source"] = f"oracle/_ref/{name if name != 'fields' else 'ref_fields'} built by oracle/ref.mk"
         (OUT / f"ref_{name}.json").write_text(json.dumps(data, indent=1, ensure_ascii=False) + "\n")
         print(f"wrote ref_{name}.json", flush=True)
 

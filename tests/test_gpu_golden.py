@@ -140,3 +140,24 @@ def test_equilibrium_moments_gpu():
         assert abs(m - rho) <= 3e-5 * max(1.0, rho)
         assert abs(px - rho * ux) <= 3e-5 and abs(py - rho * uy) <= 3e-5 and abs(pz - rho * uz) <= 3e-5
     assert not A.equilibrium_moments(1.0, float("nan"), 0.0, 0.0)[0]
+
+
+REF_FIELDS = json.loads((GOLD / "ref_fields.json").read_text())
+
+
+@pytest.mark.parametrize("case", sorted(REF_FIELDS["cases"]))
+def test_per_cell_fields_match_the_reference_solver_gpu(case):
+    """The HIP solver's fields, bit for bit, against the reference solver's own
+    per-cell dump (SHA-256 of the float32 bytes)."""
+    import hashlib
+    cfg, steps = W.reference_field_cases()[case]
+    ref = REF_FIELDS["cases"][case]
+    print(f"per-cell case {case}", flush=True)
+    L = A.Lattice(A.default_config(**cfg))
+    assert list(L.dims) == ref["grid"]
+    L.step(steps)
+    for name, digest in ref["digests"].items():
+        v = np.ascontiguousarray(L.field(name), dtype="<f4").ravel()
+        for i, want in ref["samples"][name].items():
+            assert float(v[int(i)]) == want, (name, i, float(v[int(i)]), want)
+        assert hashlib.sha256(v.tobytes()).hexdigest() == digest, name

@@ -299,3 +299,25 @@ def test_quantisers_round_to_nearest_even(oracle):
         d = [abs(float(x) - v) for v in vals]
         pick = vals[0] if d[0] < d[1] else (vals[1] if d[1] < d[0] else vals[(cand[0] >> 16) & 1])
         assert oracle.q_bf16(float(x)) == pick, x
+
+
+REF_FIELDS = json.loads((GOLD / "ref_fields.json").read_text())
+
+
+@pytest.mark.parametrize("case", sorted(REF_FIELDS["cases"]))
+def test_per_cell_fields_match_the_reference_solver(oracle, case):
+    """Every cell of every field, bit for bit, against the reference solver's
+    own dump (SHA-256 of the float32 bytes): convective outlet, LES with Van
+    Driest damping near the body, thermal coupling and the Dirichlet rows."""
+    import hashlib
+    import aeros_lbm as A
+    cfg, steps = W.reference_field_cases()[case]
+    ref = REF_FIELDS["cases"][case]
+    L = oracle.OracleLattice(A.default_config(**cfg))
+    assert list(L.dims) == ref["grid"] and steps == ref["steps"]
+    L.step(steps)
+    for name, digest in ref["digests"].items():
+        v = np.ascontiguousarray(L.field(name), dtype="<f4").ravel()
+        for i, want in ref["samples"][name].items():
+            assert float(v[int(i)]) == want, (name, i, float(v[int(i)]), want)
+        assert hashlib.sha256(v.tobytes()).hexdigest() == digest, name
