@@ -113,8 +113,9 @@ struct EngineBase {
     // A field, as float, into a device buffer on the solver stream (no wait).
     virtual int field_device(int id, float* dst) = 0;
     virtual int step_count() const = 0;   // lbmCurrentStep
-    // HIP-event timing of the step kernel launches
-    bool timing = false;
+    // HIP-event timing of the step kernel launches: 1 = a pair around every
+    // launch, 2 = one pair around the launches of each call
+    int timing = 0;
     double timed_ms = 0.0;
     int64_t timed_launches = 0;
     std::vector<hipEvent_t> events;
@@ -424,9 +425,10 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         const int start = cur;
         const bool was_pre = pre;
         if (timing) {
-            const int rc = ensure_events((size_t)nsteps * 2);
+            const int rc = ensure_events(timing == 1 ? (size_t)nsteps * 2 : 2);
             if (rc) return rc;
         }
+        if (timing == 2) HIP_TRY(hipEventRecord(events[0], st));
         for (int i = 0; i < nsteps; ++i) {
             const int k = start + i;
             P.first = (was_pre && i == 0) ? 1 : 0;
@@ -435,10 +437,11 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             P.k = k;
             P.pending = i > 0 ? 1 : 0;
             const StepBuffers B = buffers(k, P.first != 0, P.write_fields != 0);
-            if (timing) HIP_TRY(hipEventRecord(events[(size_t)i * 2], st));
+            if (timing == 1) HIP_TRY(hipEventRecord(events[(size_t)i * 2], st));
             HIP_TRY((launch_step<R, S, U>(P, B, grid, st)));
-            if (timing) HIP_TRY(hipEventRecord(events[(size_t)i * 2 + 1], st));
+            if (timing == 1) HIP_TRY(hipEventRecord(events[(size_t)i * 2 + 1], st));
         }
+        if (timing == 2) HIP_TRY(hipEventRecord(events[1], st));
         {   // fold the call's last step into the status
             const int k = start + nsteps - 1;
             hipLaunchKernelGGL((k_fold<R>), dim3(1), dim3(BLOCK), 0, st, P, buffers(k, false, false), k, (uint32_t)grid);
@@ -447,7 +450,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         int rc = finish(start, nsteps, was_pre);
         if (rc) return rc;
         if (timing) {
-            for (int i = 0; i < nsteps; ++i) {
+            for (int i = 0; i < (timing == 1 ? nsteps : 1); ++i) {
                 float ms = 0.0f;
                 HIP_TRY(hipEventElapsedTime(&ms, events[(size_t)i * 2], events[(size_t)i * 2 + 1]));
                 timed_ms += ms;
@@ -1366,8 +1369,8 @@ int aeros_lbm_kernel_timing(aeros_lbm* h, int32_t enable, double* total_ms, int6
     if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
     std::lock_guard<std::mutex> lock_(h->mu);
     EngineBase* e = h->eng.get();
-    if (enable < 0) { e->timed_ms = 0.0; e->timed_launches = 0; e->timing = false; }
-    else e->timing = enable != 0;
+    if (enable < 0) { e->timed_ms = 0.0; e->timed_launches = 0; e->timing = 0; }
+    else e->timing = enable > 2 ? 2 : enable;
     if (total_ms) *total_ms = e->timed_ms;
     if (launches) *launches = e->timed_launches;
     return AEROS_OK;

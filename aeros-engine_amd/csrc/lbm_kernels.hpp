@@ -99,11 +99,8 @@ template <> struct Pop<_Float16, float> {   // deviation from w_q, IEEE half, ro
     __device__ static void st(_Float16* p, uint32_t i, float v, float w) { st_nt(at(p, i), half_minus(v, w)); }
 };
 __device__ inline float bf16_to_f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
-__device__ inline uint16_t f_to_bf16(float x) {   // round to nearest even
-    uint32_t u = __float_as_uint(x);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)(u >> 16);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (uint16_t)(u >> 16);
+__device__ inline uint16_t f_to_bf16(float x) {   // round to nearest even (one conversion on gfx950)
+    return __builtin_bit_cast(uint16_t, (__bf16)x);
 }
 template <> struct Pop<bf16_t, float> {     // deviation from w_q, bfloat16
     __device__ static float ld(const bf16_t* p, uint32_t i, float w) { return w + bf16_to_f(at(p, i).bits); }

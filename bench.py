@@ -9,8 +9,9 @@ seeded generator) voxelized by the BVH three-ray vote at the cell centres.
 The lattice and its mask are resident in HBM before the timed region.
 
 Prints one JSON line.  `value` is MLUPS = cells * steps / seconds / 1e6 over
-the timed steps; `roofline` uses the step kernel's average duration from HIP
-events on the solver's stream and the algorithmic 177 bytes per cell update
+the timed steps; `roofline` uses the step kernel's average duration from one
+HIP event pair on the solver's stream around the timed back-to-back launches
+(an upper bound: it includes the dispatch gaps) and the algorithmic 177 bytes per cell update
 (19 FP32 loads + 19 stores + previous-step velocity read and current write for
 the Smagorinsky stencil + 1 flag byte).  `roofline.traffic` is the HBM bytes
 per step-kernel launch from two rocprofv3 counter passes (FETCH_SIZE, then
@@ -59,15 +60,14 @@ def run_gpu(A, cfg, steps, warmup, label, T=None):
         L.thermal_upload(T)
     log(f"[{label}] lattice {L.dims} ready in {time.time() - t0:.1f} s")
     L.step(warmup)
-    # throughput: the K steps alone (an event pair around every launch would
-    # hold back the next dispatch and lower the rate by 4-20 % on these sizes)
+    # The K timed steps; the step kernel's duration from one HIP event pair on
+    # the solver's stream around the K back-to-back launches (a pair around
+    # every launch would itself hold back each next dispatch by several us)
+    L.kernel_timing(-1)
+    L.kernel_timing(2)
     t1 = time.perf_counter()
     L.step(steps)
     dt = time.perf_counter() - t1
-    # step-kernel duration: HIP events around each launch on the solver's stream
-    L.kernel_timing(-1)
-    L.kernel_timing(1)
-    L.step(min(steps, 300))
     kms, launches = L.kernel_timing(0)
     sc = L.scalars()
     cells = L.ncells
@@ -356,6 +356,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_step<float,float,float,LES>", "kernel_avg_ms": round(r["kernel_ms"], 5),
+                     "timing": "one HIP event pair around the timed launches on the solver stream",
                      "bytes_per_cell": bpc},
         "pct_hbm_from_mlups": round(r["mlups"] * 1e6 * bpc / (HBM_PEAK_GBS * 1e9), 4),
         "voxelize": {"mesh": "199,080-triangle jittered ellipsoid on 256x128x104",

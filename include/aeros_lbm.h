@@ -449,9 +449,11 @@ int aeros_voxel_plan(const float bb_min[3], const float bb_max[3], float max_dim
 /* Kernel timing of the step kernel (no counterpart in the reference, whose
  * only timer is the wall time of a stepLBMCPU call, aeros/src/lbm.cpp:916,1707):
  * enable = 1 brackets every step-kernel launch with HIP events on the
- * handle's stream and accumulates their elapsed time; enable = 0 stops and
- * leaves the totals readable; enable = -1 resets them.  total_ms and
- * launches may be NULL. */
+ * handle's stream and accumulates their elapsed time; enable = 2 brackets
+ * the back-to-back launches of each step call with one event pair (the
+ * per-launch pair itself delays the next dispatch by several microseconds);
+ * enable = 0 stops and leaves the totals readable; enable = -1 resets them.
+ * total_ms and launches may be NULL. */
 int aeros_lbm_kernel_timing(aeros_lbm* h, int32_t enable, double* total_ms, int64_t* launches);
 
 /* cudaConsumeError analogue (aeros/src/cuda_api.h:53-56): copies the last
