@@ -1019,11 +1019,13 @@ struct BlockMap {
 };
 
 template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
-__global__ __launch_bounds__(BLOCK) void k_step_batch(const BatchCase<R>* cases, const BlockMap* map, int i,
-                                                      int nsteps) {
+__global__ __launch_bounds__(BLOCK) void k_step_batch(const BatchCase<R>* __restrict__ cases,
+                                                      const BlockMap* __restrict__ map, int i, int nsteps) {
     __shared__ RedLds L;
+    // block-uniform, read-only: scalar loads, so the case's parameters sit in
+    // scalar registers as a kernel argument's would
     const BlockMap bm = map[blockIdx.x];
-    const BatchCase<R>& C = cases[bm.ci];
+    const BatchCase<R>& C = cases[__builtin_amdgcn_readfirstlane(bm.ci)];
     const int k = C.start + i;
     const StepFlags F{(C.pre && i == 0) ? 1 : 0, (i == nsteps - 1) ? 1 : 0, 0, k, i > 0 ? 1 : 0};
     StepBuffers B = C.buf[k & 1];
