@@ -542,10 +542,12 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
             return k_sqrt<S, R>(R(2) * s2);
         };
         if (EARLY_U && strain) load_u();
-        // likewise the shifted temperature populations and the buoyancy of an
-        // interior cell (solid sources are replaced after the collision)
+        // the shifted temperature populations and the buoyancy of an interior
+        // cell join the population loads with every storage type (solid
+        // sources are replaced after the collision); the thermal kernels keep
+        // 4 waves per SIMD either way
         R gq[Q7], bq = R(0);
-        const bool early_g = THERM && EARLY_U && !(m & M_FACES);
+        const bool early_g = THERM && !(m & M_FACES);
         if (early_g) {
             const R* g = static_cast<const R*>(B.g_in);
 #pragma unroll
