@@ -301,4 +301,7 @@ def reference_field_cases() -> dict:
         "bench16": (bench_sphere(16, les=False), 150),
         "rb_walls": (thermal_box(32, 18, 8, walls_y=True, rayleigh=2500.0, fixed_wall_temps=True,
                                  perturbation=0.02), 200),
+        # buoyancy large enough to overflow the populations: the NaN-storm reset
+        "rb_storm": (thermal_box(32, 18, 8, walls_y=True, rayleigh=1.0e30, fixed_wall_temps=True,
+                                 perturbation=0.02), 12),
     }

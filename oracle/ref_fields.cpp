@@ -11,7 +11,7 @@
 // unit, built with its main renamed; this file adds none.
 //
 //   oracle/_ref/ref_fields <case> <out-dir>
-//   cases: bench12_les  bench12_les_thermal  bench16  rb_walls
+//   cases: bench12_les  bench12_les_thermal  bench16  rb_walls  rb_storm
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -119,16 +119,21 @@ int main(int argc, char** argv) {
     else if (which == "bench12_les_thermal") { sphere_case(12, true, true); steps = 90; }
     else if (which == "bench16") { sphere_case(16, false, false); steps = 150; }
     else if (which == "rb_walls") { rb_case(); steps = 200; }
+    else if (which == "rb_storm") {   // buoyancy so large that the populations overflow: the NaN-storm reset
+        rb_case();
+        thermalParams.rayleigh = 1.0e30f;
+        steps = 12;
+    }
     else { std::fprintf(stderr, "unknown case %s\n", which.c_str()); return 2; }
     shutdownLBM();
     initLBM();
     if (!lbmInitialized) { std::fprintf(stderr, "init failed\n"); return 1; }
-    for (int i = 0; i < steps; ++i) stepLBMCPU(1);
+    for (int i = 0; i < steps; ++i) stepLBMCPU(1);   // one call per step, as the frame loop does
     bool ok = dump(dir, "rho", lbmRho) && dump(dir, "ux", lbmUx) && dump(dir, "uy", lbmUy) && dump(dir, "uz", lbmUz) &&
               dump(dir, "tke", lbmTKEField) && dump(dir, "strain", lbmStrainMag) && dump(dir, "ux_world", lbmUxWorld) &&
               dump(dir, "pressure", lbmPressure);
     if (ok && thermalParams.enabled) ok = dump(dir, "T", thermalT) && dump(dir, "buoyancy", thermalBuoyancyY);
-    std::printf("%s %dx%dx%d steps %d\n", which.c_str(), lbmNx, lbmNy, lbmNz, steps);
+    std::printf("%s %dx%dx%d steps %d step %d\n", which.c_str(), lbmNx, lbmNy, lbmNz, steps, lbmCurrentStep);
     shutdownLBM();
     return ok ? 0 : 1;
 }

@@ -147,7 +147,7 @@ def lbm_suite() -> dict:
 # Per-cell field dumps of oracle/_ref/ref_fields (the reference solver run on
 # small cases): recorded as SHA-256 digests of the raw float32 bytes, plus a
 # few values for a readable failure.
-FIELD_CASES = ["bench12_les", "bench12_les_thermal", "bench16", "rb_walls"]
+FIELD_CASES = ["bench12_les", "bench12_les_thermal", "bench16", "rb_walls", "rb_storm"]
 
 
 def fields() -> dict:
@@ -158,8 +158,9 @@ def fields() -> dict:
         d = ROOT / "oracle" / "_ref" / "fields" / case
         d.mkdir(parents=True, exist_ok=True)
         text = run("ref_fields", case, str(d))
-        grid, steps = re.search(r"(\d+)x(\d+)x(\d+) steps (\d+)", text).groups()[:3], int(re.search(r"steps (\d+)", text).group(1))
-        rec = {"grid": [int(g) for g in grid], "steps": steps, "digests": {}, "samples": {}}
+        m = re.search(r"(\d+)x(\d+)x(\d+) steps (\d+) step (\d+)", text)
+        rec = {"grid": [int(g) for g in m.groups()[:3]], "steps": int(m.group(4)), "calls": "one per step",
+               "final_step": int(m.group(5)), "digests": {}, "samples": {}}
         for f in sorted(d.glob("*.f32")):
             raw = f.read_bytes()
             v = np.frombuffer(raw, dtype="<f4")

@@ -155,7 +155,9 @@ def test_per_cell_fields_match_the_reference_solver_gpu(case):
     print(f"per-cell case {case}", flush=True)
     L = A.Lattice(A.default_config(**cfg))
     assert list(L.dims) == ref["grid"]
-    L.step(steps)
+    for _ in range(steps):                 # one call per step, as the reference's frame loop
+        L.step(1)
+    assert L.scalars()["step"] == ref["final_step"]   # calls broken by a reset do not advance it
     for name, digest in ref["digests"].items():
         v = np.ascontiguousarray(L.field(name), dtype="<f4").ravel()
         for i, want in ref["samples"][name].items():
