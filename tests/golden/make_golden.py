@@ -171,15 +171,41 @@ def fields() -> dict:
     return out
 
 
+def voxels() -> dict:
+    """The reference's own vote on the synthetic ellipsoid STLs of the measured cases (the
+    199,080-triangle C2 body on 256x128x104 and its x2 scale on 512x256x256), as SHA-256
+    digests of the uint8 masks."""
+    import hashlib
+    sys.path.insert(0, str(ROOT / "aeros-engine_amd"))
+    import workloads as W
+    out = {"cases": {}}
+    d = ROOT / "oracle" / "_ref" / "voxels"
+    d.mkdir(parents=True, exist_ok=True)
+    for name, scale in (("c2", 1.0), ("c4", 2.0)):
+        meshes = {}
+        cfg = W.c2_config(lambda t, dims, mn, cs: meshes.setdefault("m", (t, dims, mn, cs)) and None,
+                          scale=scale, with_mask=True)
+        tris, dims, mn, cs = meshes["m"]
+        (d / f"{name}.f32").write_bytes(tris.astype("<f4").tobytes())
+        text = run("ref_voxels", str(d / f"{name}.f32"), *[str(x) for x in dims], *[repr(float(x)) for x in mn],
+                   *[repr(float(x)) for x in cs], str(d / f"{name}.u8"))
+        ntri, solid = (int(x) for x in re.search(r"triangles (\d+) solid (\d+)", text).groups())
+        raw = (d / f"{name}.u8").read_bytes()
+        out["cases"][name] = {"grid": list(dims), "min": list(mn), "cell": list(cs), "triangles": ntri,
+                              "solid": solid, "sha256": hashlib.sha256(raw).hexdigest()}
+        del cfg
+    return out
+
+
 def main() -> None:
     if not (REF / "bench").exists():
         raise SystemExit("oracle/_ref is not built: make -f oracle/ref.mk -j8")
-    which = sys.argv[1:] or ["bench", "physref", "thermal", "validate", "lbm", "fields"]
+    which = sys.argv[1:] or ["bench", "physref", "thermal", "validate", "lbm", "fields", "voxels"]
     funcs = {"bench": bench, "physref": physref, "thermal": thermal, "validate": validate, "lbm": lbm_suite,
-             "fields": fields}
+             "fields": fields, "voxels": voxels}
     for name in which:
         data = funcs[name]()
-        data["_source"] = f"oracle/_ref/{name if name != 'fields' else 'ref_fields'} built by oracle/ref.mk"
+        data["_source"] = f"oracle/_ref/{ {'fields': 'ref_fields', 'voxels': 'ref_voxels'}.get(name, name)} built by oracle/ref.mk"
         (OUT / f"ref_{name}.json").write_text(json.dumps(data, indent=1, ensure_ascii=False) + "\n")
         print(f"wrote ref_{name}.json", flush=True)
 

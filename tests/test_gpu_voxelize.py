@@ -65,3 +65,17 @@ def test_empty_degenerate_and_non_finite_meshes():
     a = check(np.concatenate([box, nan_tri]), dims, (0, 0, 0), (1, 1, 1))
     b = A.voxelize(box, dims, (0, 0, 0), (1, 1, 1))
     np.testing.assert_array_equal(a, b)
+
+
+def test_device_vote_matches_the_reference_on_the_ellipsoid_stls():
+    """The device voxelizer against the reference's own vote (SHA-256 of the
+    uint8 masks of the C2 body and its x2 scale)."""
+    import hashlib
+    import json
+    from pathlib import Path
+    ref = json.loads((Path(__file__).resolve().parent / "golden" / "ref_voxels.json").read_text())["cases"]
+    for case, scale in (("c2", 1.0), ("c4", 2.0)):
+        print(f"reference vote {case}", flush=True)
+        m = np.ascontiguousarray(W.c2_config(A.voxelize_gpu, scale=scale)["voxels"]).astype(np.uint8)
+        assert int(m.sum()) == ref[case]["solid"]
+        assert hashlib.sha256(m.tobytes()).hexdigest() == ref[case]["sha256"], case

@@ -188,3 +188,18 @@ def test_server_without_device_fails_cleanly():
     lib.aeros_serve_config_defaults(C.byref(bad))
     bad.struct_size = 8
     assert not lib.aeros_serve_create(C.byref(bad), C.byref(err)) and err.value == -1
+
+
+REF_VOXELS = __import__("json").loads((ROOT / "tests" / "golden" / "ref_voxels.json").read_text())
+
+
+@pytest.mark.parametrize("case,scale", [("c2", 1.0), ("c4", 2.0)])
+def test_voxelizer_matches_the_reference_vote_on_the_ellipsoid_stl(case, scale):
+    """The host voxelizer's mask of the synthetic ellipsoid STL equals, voxel for
+    voxel, the reference's own BVH vote on the same triangles and grid
+    (SHA-256 of the uint8 mask, tests/golden/ref_voxels.json)."""
+    import hashlib
+    ref = REF_VOXELS["cases"][case]
+    m = np.ascontiguousarray(W.c2_config(A.voxelize, scale=scale)["voxels"]).astype(np.uint8)
+    assert list(m.shape[::-1]) == ref["grid"] and int(m.sum()) == ref["solid"]
+    assert hashlib.sha256(m.tobytes()).hexdigest() == ref["sha256"]
