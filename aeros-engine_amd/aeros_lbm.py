@@ -292,6 +292,7 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
         "aeros_lbm_read_populations": (C.c_int, [vp, P(C.c_double)]),
         "aeros_lbm_cv_force": (C.c_int, [vp, C.c_float, C.c_float, C.c_float, P(C.c_double)]),
         "aeros_lbm_projected_area": (C.c_int, [vp, C.c_int32, P(C.c_float)]),
+        "aeros_lbm_rho_sum": (C.c_int, [vp, P(C.c_double)]),
         "aeros_lbm_checksums": (C.c_int, [vp, P(C.c_double), P(C.c_double)]),
         "aeros_lbm_thermal_upload": (C.c_int, [vp, P(C.c_float)]),
         "aeros_lbm_thermal_reset": (C.c_int, [vp]),
@@ -548,6 +549,11 @@ class Lattice:
         a, b = C.c_double(), C.c_double()
         _check(self.lib.aeros_lbm_checksums(self.h, C.byref(a), C.byref(b)), "aeros_lbm_checksums")
         return a.value, b.value
+
+    def rho_sum(self) -> float:
+        a = C.c_double()
+        _check(self.lib.aeros_lbm_rho_sum(self.h, C.byref(a)), "aeros_lbm_rho_sum")
+        return a.value
 
     def thermal_upload(self, T):
         T = np.ascontiguousarray(T, dtype=np.float32).ravel()

@@ -1106,6 +1106,11 @@ int aeros_lbm_checksums(aeros_lbm* h, double* rho_sum, double* u_sum) {
     return h->eng->checksums(rho_sum, u_sum);
 }
 
+int aeros_lbm_rho_sum(aeros_lbm* h, double* rho_sum) {
+    double u_sum = 0.0;
+    return aeros_lbm_checksums(h, rho_sum, &u_sum);
+}
+
 int aeros_lbm_thermal_upload(aeros_lbm* h, const float* T) {
     if (!h || !h->eng || !T) return set_err(AEROS_ERR_ARG, "null argument");
     std::lock_guard<std::mutex> lock_(h->mu);

@@ -334,6 +334,9 @@ int aeros_lbm_projected_area(aeros_lbm* h, int32_t axis, float* out);
 /* Deterministic FP64 sums over all cells of rho and of ux+uy+uz, the bench's
  * checksum (aeros/tests/bench_lbm.cpp:103-107). */
 int aeros_lbm_checksums(aeros_lbm* h, double* rho_sum, double* u_sum);
+/* The deterministic FP64 mass (rho sum over all cells) alone, the quantity
+ * the FP64 acceptance criterion compares over 120 steps. */
+int aeros_lbm_rho_sum(aeros_lbm* h, double* rho_sum);
 
 /* ---------------------------------------------------------------- thermal */
 
