@@ -95,3 +95,37 @@ def test_lattice_dt_world():
     # cell size / (flowSpeed / |u_in|): bench sphere, 20 m/s over U0 0.08
     dx = F.cell_size[0]
     assert F.lattice_dt() == pytest.approx(dx / (20.0 / 0.08), rel=1e-6)
+
+
+def test_calls_from_another_thread_while_frames_run():
+    """A second host thread reads fields and scalars while the frame worker
+    steps: every call is serialised with the frame in progress, and the end
+    state equals the same updates made synchronously."""
+    import threading
+    F, S = twins()
+    F.frames_start(("rho", "ux"))
+    errors, reads = [], []
+
+    def reader():
+        try:
+            for _ in range(30):
+                reads.append(F.scalars()["step"])
+                F.field("uy")
+        except Exception as e:          # pragma: no cover - reported below
+            errors.append(e)
+
+    t = threading.Thread(target=reader)
+    t.start()
+    for _ in range(12):
+        F.frame_submit(0.016, 7, 3)
+    F.frames_wait()
+    t.join()
+    assert not errors, errors
+    assert reads == sorted(reads)      # the step count a reader sees never goes back
+    for _ in range(12):
+        S.update(0.016, 7, 3)
+    fr = F.frame_latest()
+    assert fr["frame"] == 12 and fr["scalars"]["step"] == 84
+    np.testing.assert_array_equal(fr["fields"]["ux"], S.field("ux"))
+    np.testing.assert_array_equal(F.field("uy"), S.field("uy"))
+    F.frames_stop()
