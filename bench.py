@@ -123,6 +123,29 @@ def cpu_baseline(cfg, budget_s=12.0, max_steps=40):
                        f"oracle (C++ restatement, OpenMP {threads} threads)")
 
 
+def cpu_others(A, W, budget_s=6.0):
+    """The restatement on the other configurations, a few seconds each: C1 in FP64, the
+    coupled thermal C3, and C4 in FP32 (the reference itself stops at 20 Mi cells)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    out = {}
+    c3, c3T = W.c3_rayleigh_benard()
+    for name, cfg, T in (("c1_fp64", W.c1_sphere_fp64(), None), ("c3_thermal_fp32", c3, c3T),
+                         ("c4_fp32", W.c2_config(A.voxelize, scale=2.0), None)):
+        L = O.OracleLattice(O.default_config(**cfg))
+        if T is not None:
+            L.thermal_upload(T)
+        L.step(1)
+        n, t0 = 0, time.perf_counter()
+        while n < 40 and (n < 2 or time.perf_counter() - t0 < budget_s):
+            L.step(1)
+            n += 1
+        out[name] = {"mlups": round(L.ncells * n / (time.perf_counter() - t0) / 1e6, 3), "steps": n}
+        L.close()
+        log(f"[cpu] {name}: {out[name]}")
+    return out
+
+
 # ------------------------------------------------------------------ served mode
 def solve_alone(A, W, cfg, round_steps, tol, max_steps):
     """One request on its own lattice: rounds of round_steps, drag after each, the
@@ -401,6 +424,8 @@ def main():
             out["roofline"]["traffic_note"] = err
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(c2)
+        if not args.no_secondary:
+            out["cpu_baseline"]["others"] = cpu_others(A, W)
     print(json.dumps(out), flush=True)
 
 
