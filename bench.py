@@ -8,8 +8,15 @@ Workload (one step = one D3Q19 collide+stream pass over the whole lattice):
 seeded generator) voxelized by the BVH three-ray vote at the cell centres.
 The lattice and its mask are resident in HBM before the timed region.
 
-Prints one JSON line.  `value` is MLUPS = cells * steps / seconds / 1e6 over
-the timed steps; `roofline` uses the step kernel's average duration from one
+Each timed run warms up, restarts the lattice from its initial state and times
+steps 1..K of the case.  The case diverges after about 450 steps (as the
+reference solver does on it: open side faces, tau 0.5192) and its step loop
+breaks after 980 steps; the launches after the break return at once, so
+throughput and kernel time count only the steps the call advanced
+(`steps_done`).
+
+Prints one JSON line.  `value` is MLUPS = cells * steps_done / seconds / 1e6
+over the timed call; `roofline` uses the step kernel's average duration from one
 HIP event pair on the solver's stream around the timed back-to-back launches
 (an upper bound: it includes the dispatch gaps) and the algorithmic 177 bytes per cell update
 (19 FP32 loads + 19 stores + previous-step velocity read and current write for
@@ -18,7 +25,9 @@ per step-kernel launch from two rocprofv3 counter passes (FETCH_SIZE, then
 WRITE_SIZE) over a short run of the same case, each scaled by what the same
 counter reads for a plain copy kernel of known size in the same process.
 `secondary` holds the other precisions and cases (33.5M cells FP32/FP16/BF16,
-the FP64 sphere, the coupled thermal lattice); `served` the served-mode
+the FP64 sphere, the coupled thermal lattice), the 33.5M-cell precision study
+(Cd and velocity L2 of FP32/FP16/BF16 against FP64) and the Rayleigh-Benard
+Nusselt number of FP32/FP16 against FP64 at 300k steps; `served` the served-mode
 stream (solves/s, p50/p99 latency, against one-at-a-time solves);
 `cpu_baseline` times the CPU restatement on the host cores for a bounded
 number of steps of the same case.
@@ -54,12 +63,21 @@ def log(msg):
 
 
 def run_gpu(A, cfg, steps, warmup, label, T=None):
+    """W warm-up steps, then the lattice restarts from its initial state and the K
+    timed steps are steps 1..K of the case.  A step that breaks the reference's step
+    loop (NaN storm, or mean rho outside 0.8..1.2) ends the call there and the later
+    launches of the call return at once; throughput and the kernel average are taken
+    over the steps the call really advanced."""
     t0 = time.time()
     L = A.Lattice(A.default_config(**cfg))
     if T is not None:
         L.thermal_upload(T)
     log(f"[{label}] lattice {L.dims} ready in {time.time() - t0:.1f} s")
     L.step(warmup)
+    L.reset()
+    if T is not None:
+        L.thermal_upload(T)
+    sc0 = L.scalars()
     # The K timed steps; the step kernel's duration from one HIP event pair on
     # the solver's stream around the K back-to-back launches (a pair around
     # every launch would itself hold back each next dispatch by several us)
@@ -70,12 +88,15 @@ def run_gpu(A, cfg, steps, warmup, label, T=None):
     dt = time.perf_counter() - t1
     kms, launches = L.kernel_timing(0)
     sc = L.scalars()
+    done = sc["step"] - sc0["step"]
+    resets = sc["resets"] - sc0["resets"]
     cells = L.ncells
-    res = dict(dims=L.dims, cells=cells, seconds=dt, mlups=cells * steps / dt / 1e6,
-               kernel_ms=kms / max(1, launches), launches=launches, avg_rho=sc["avg_rho"],
-               max_u=sc["max_u_lb"], step=sc["step"], nusselt=sc.get("nusselt"))
+    res = dict(dims=L.dims, cells=cells, seconds=dt, mlups=cells * done / dt / 1e6,
+               kernel_ms=kms / max(1, done), launches=launches, steps_done=done, resets=resets,
+               avg_rho=sc["avg_rho"], max_u=sc["max_u_lb"], step=sc["step"], nusselt=sc.get("nusselt"))
     L.close()
-    log(f"[{label}] {res['mlups']:.1f} MLUPS, step kernel {res['kernel_ms']:.5f} ms")
+    log(f"[{label}] {res['mlups']:.1f} MLUPS, step kernel {res['kernel_ms']:.5f} ms, {done} of {steps} steps "
+        f"advanced, {resets} reset(s), max |u| {sc['max_u_lb']:.4f}")
     return res
 
 
@@ -95,8 +116,80 @@ def c1_results(A, W):
     return {"rho_sum_120": float("%.12g" % rho_sum), "cd_2000": round(cd, 4), "area_cells": area}
 
 
+def c3_nusselt(A, W, steps=300000, chunk=20000, precisions=("fp64", "fp32", "fp16")):
+    """The C3 Rayleigh-Benard box run to steady state in each precision from the same
+    seeded start: Nu at `steps` steps and its change over the last `chunk` steps.  The
+    FP64 run stands for the FP64 oracle, which the FP64 kernel matches within 1e-12 and
+    which cannot itself run 1.05M cells for 300k steps on the host."""
+    out = {}
+    for prec in precisions:
+        cfg, T = W.c3_rayleigh_benard()
+        cfg["precision"] = prec
+        L = A.Lattice(A.default_config(**cfg))
+        L.thermal_upload(T)
+        t0 = time.perf_counter()
+        nus = []
+        for _ in range(steps // chunk):
+            L.step(chunk)
+            nus.append(float(L.scalars()["nusselt"]))
+        sc = L.scalars()
+        L.close()
+        out[prec] = {"nu": round(nus[-1], 6), "dnu_last_chunk": float("%.3g" % (nus[-1] - nus[-2])),
+                     "max_u": float("%.4g" % sc["max_u_lb"]), "steps": sc["step"],
+                     "seconds": round(time.perf_counter() - t0, 1)}
+        log(f"[c3 nu] {prec}: {out[prec]}")
+    ref = out["fp64"]["nu"]
+    for prec in precisions[1:]:
+        out[prec]["rel_err_vs_fp64"] = float("%.4g" % ((out[prec]["nu"] - ref) / ref))
+    out["note"] = (f"{steps} steps of 128x64x128, Ra 2500, Pr 0.71, tau 0.6, T linear + seeded U(-1e-3, 1e-3); "
+                   "Nu sampled every %d steps" % chunk)
+    return out
+
+
+def c4_precision_study(A, W, steps=6000, window=2000, every=100):
+    """C4's body, grid and Re in FP64, FP32, FP16 and BF16 storage from the same start:
+    the drag coefficient (control volume 0.45/0.40/0.40 of the domain, A = projected
+    solid cells along x) averaged over samples every `every` steps in the last `window`
+    steps, and the velocity field's relative L2 over the fluid cells after `steps`, each
+    against the FP64 run.  With the reference's open side faces this flow diverges after
+    about 450 steps in every precision (the reference solver's own run of the 256x128x104
+    case does the same), so the study closes the y and z faces with free-slip walls."""
+    cfg0 = W.c2_config(A.voxelize_gpu, scale=2.0)
+    cfg0["free_slip"] = W.SIDE_WALLS
+    fluid = (np.asarray(cfg0["voxels"]) == 0).reshape(-1)
+    out, u64, cd64 = {}, None, None
+    for prec in ("fp64", "fp32", "fp16", "bf16"):
+        cfg = dict(cfg0, precision=prec)
+        L = A.Lattice(A.default_config(**cfg))
+        area = L.projected_area(0)
+        t0 = time.perf_counter()
+        L.step(steps - window)
+        cds = []
+        for _ in range(window // every):
+            L.step(every)
+            cds.append(L.cv_force(0.45, 0.40, 0.40)[0] / (0.5 * 0.08 * 0.08 * area))
+        u = np.stack([L.field(k).reshape(-1)[fluid].astype(np.float64) for k in ("ux", "uy", "uz")])
+        sc = L.scalars()
+        L.close()
+        cd = float(np.mean(cds))
+        rec = {"cd_mean": round(cd, 5), "cd_last": round(cds[-1], 5), "steps": sc["step"], "resets": sc["resets"],
+               "seconds": round(time.perf_counter() - t0, 1)}
+        if u64 is None:
+            u64, cd64 = u, cd
+        else:
+            rec["cd_rel_err_vs_fp64"] = float("%.4g" % ((cd - cd64) / cd64))
+            rec["u_rel_l2_vs_fp64"] = float("%.4g" % np.sqrt(((u - u64) ** 2).sum() / (u64 ** 2).sum()))
+        out[prec] = rec
+        log(f"[c4 precision] {prec}: {rec}")
+    out["note"] = (f"512x256x256, ellipsoid 60/24/24, Re 1000 (tau 0.51152), TRT+LES, convective outlet, "
+                   f"free-slip y/z faces; Cd mean of {window // every} samples over steps "
+                   f"{steps - window + every}..{steps}; L2 after {steps} steps")
+    return out
+
+
 def line(r, bpc):
     return {"mlups": round(r["mlups"], 1), "kernel_ms": round(r["kernel_ms"], 5),
+            "steps_done": r["steps_done"], "resets": r["resets"],
             "pct_hbm": round(r["mlups"] * 1e6 * bpc / (HBM_PEAK_GBS * 1e9), 4),
             "achieved_gbs": round(bpc * r["cells"] / (r["kernel_ms"] * 1e-3) / 1e9, 1),
             "bytes_per_cell": bpc, "cells": r["cells"]}
@@ -367,7 +460,9 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r["seconds"] * 1e3 / args.steps, 4),
+        "ms_per_step": round(r["seconds"] * 1e3 / max(1, r["steps_done"]), 4),
+        "steps_done": r["steps_done"],
+        "resets": r["resets"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -375,7 +470,8 @@ def main():
         "data": "synthetic (seeded jittered ellipsoid STL, voxelized by the BVH three-ray vote)",
         "config": {"workload": "256x128x104 D3Q19 TRT(3/16)+Smagorinsky Cs0.1 FP32, ellipsoid 30/12/12 cells, "
                                "U0 0.08, tau 0.5192, convective outlet",
-                   "cells": r["cells"], "parallelism": "single GPU"},
+                   "cells": r["cells"], "parallelism": "single GPU",
+                   "timed": "steps 1..K of the case: W warm-up steps, then a restart from the initial state"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_step<float,float,float,LES>", "kernel_avg_ms": round(r["kernel_ms"], 5),
@@ -402,6 +498,8 @@ def main():
                                   "free-slip side walls",
                             "c3": "128x64x128 coupled D3Q19+D3Q7 FP32, Rayleigh-Benard Ra 2500 Pr 0.71"}
         sec["host_driver"] = host_driver_leg(A, c2)
+        sec["c4_precision_study"] = c4_precision_study(A, W)
+        sec["c3_nusselt"] = c3_nusselt(A, W)
         out["secondary"] = sec
     if not args.no_served:
         out["served"] = served_leg(A, W)

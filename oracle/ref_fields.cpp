@@ -12,6 +12,14 @@
 //
 //   oracle/_ref/ref_fields <case> <out-dir>
 //   cases: bench12_les  bench12_les_thermal  bench16  rb_walls  rb_storm
+//
+//   oracle/_ref/ref_fields box_les <out-dir> <nx> <ny> <nz> <tau> <steps> <dump-step>
+//   a lattice-aligned box (padding 0) with the solid mask read from
+//   <out-dir>/mask.u8 (nx*ny*nz bytes, x fastest), TRT + Smagorinsky Cs 0.1,
+//   U0 0.08 along +x, convective outlet; prints the control-volume drag
+//   coefficient (box 0.45/0.40/0.40) every 50 steps and dumps the fields after
+//   <dump-step> steps.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -98,6 +106,40 @@ void rb_case() {
     thermalParams.perturbation = 0.02f;
 }
 
+// A lattice-aligned box over the caller's mask, with the bench ellipsoid's
+// flow settings (TRT + Smagorinsky, U0 0.08, convective outlet).
+bool box_case(const std::string& dir, int nx, int ny, int nz, float tau) {
+    g_voxNx = nx; g_voxNy = ny; g_voxNz = nz;
+    g_voxMinX = g_voxMinY = g_voxMinZ = 0.0f;
+    g_voxMaxX = (float)nx; g_voxMaxY = (float)ny; g_voxMaxZ = (float)nz;
+    const size_t n = (size_t)nx * ny * nz;
+    g_voxelData.assign(n, 0);
+    g_distanceField.assign(n, 0.0f);
+    std::vector<unsigned char> m(n);
+    FILE* f = std::fopen((dir + "/mask.u8").c_str(), "rb");
+    if (!f) return false;
+    const bool ok = std::fread(m.data(), 1, n, f) == n;
+    std::fclose(f);
+    if (!ok) return false;
+    for (size_t i = 0; i < n; ++i) g_voxelData[i] = m[i] ? 1 : 0;
+    minBB = glm::vec3(0.0f);
+    maxBB = glm::vec3((float)nx, (float)ny, (float)nz);
+    center = 0.5f * maxBB;
+    maxDim = (float)nx;
+    voxelResolution = nx;
+    flowSpeed = 20.0f; flowAzimuth = 0.0f; flowElevation = 0.0f;
+    altitude = 0.0f; timeScale = 1.0f; aeroMachEffects = false;
+    lbmParams.enabled = true; lbmParams.useGPU = false;
+    lbmParams.U0 = 0.08f; lbmParams.stepsPerFrame = 1; lbmParams.tau = tau;
+    lbmParams.useTurbulence = true; lbmParams.smagorinskyC = 0.1f; lbmParams.useTRT = true;
+    lbmParams.useGround = false; lbmParams.inletTurbulence = 0.0f;
+    lbmParams.domainPadding = 0.0f; lbmParams.autoViscosity = false;
+    lbmParams.useConvectiveOutlet = true; lbmParams.periodicX = false; lbmParams.periodicZ = false;
+    lbmParams.bodyForceX = lbmParams.bodyForceY = lbmParams.bodyForceZ = 0.0f;
+    thermalParams.enabled = false;
+    return true;
+}
+
 bool dump(const std::string& dir, const char* name, const std::vector<float>& v) {
     const std::string path = dir + "/" + name + ".f32";
     FILE* f = std::fopen(path.c_str(), "wb");
@@ -114,8 +156,17 @@ int main(int argc, char** argv) {
         return 2;
     }
     const std::string which = argv[1], dir = argv[2];
-    int steps = 0;
-    if (which == "bench12_les") { sphere_case(12, true, false); steps = 90; }
+    int steps = 0, dump_step = -1;
+    if (which == "box_les") {
+        if (argc < 9) { std::fprintf(stderr, "box_les needs nx ny nz tau steps dump-step\n"); return 2; }
+        if (!box_case(dir, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), (float)std::atof(argv[6]))) {
+            std::fprintf(stderr, "cannot read %s/mask.u8\n", dir.c_str());
+            return 1;
+        }
+        steps = std::atoi(argv[7]);
+        dump_step = std::atoi(argv[8]);
+    }
+    else if (which == "bench12_les") { sphere_case(12, true, false); steps = 90; }
     else if (which == "bench12_les_thermal") { sphere_case(12, true, true); steps = 90; }
     else if (which == "bench16") { sphere_case(16, false, false); steps = 150; }
     else if (which == "rb_walls") { rb_case(); steps = 200; }
@@ -128,6 +179,32 @@ int main(int argc, char** argv) {
     shutdownLBM();
     initLBM();
     if (!lbmInitialized) { std::fprintf(stderr, "init failed\n"); return 1; }
+    auto dump_all = [&]() {
+        bool ok = dump(dir, "rho", lbmRho) && dump(dir, "ux", lbmUx) && dump(dir, "uy", lbmUy) && dump(dir, "uz", lbmUz) &&
+                  dump(dir, "tke", lbmTKEField) && dump(dir, "strain", lbmStrainMag) && dump(dir, "ux_world", lbmUxWorld) &&
+                  dump(dir, "pressure", lbmPressure);
+        if (ok && thermalParams.enabled) ok = dump(dir, "T", thermalT) && dump(dir, "buoyancy", thermalBuoyancyY);
+        return ok;
+    };
+    if (dump_step >= 0) {   // box_les: drag along the way, fields at dump_step
+        const float area = lbmProjectedAreaCells(0);
+        bool ok = true;
+        for (int i = 1; i <= steps; ++i) {
+            stepLBMCPU(1);
+            if (i % 50 == 0) {
+                const glm::vec3 F = lbmControlVolumeForce(0.45f, 0.40f, 0.40f);
+                float umax = 0.0f;
+                for (size_t c = 0; c < lbmUx.size(); ++c)
+                    umax = std::max(umax, std::sqrt(lbmUx[c] * lbmUx[c] + lbmUy[c] * lbmUy[c] + lbmUz[c] * lbmUz[c]));
+                std::printf("call %d step %d Cd %.6f A %.0f max|u| %.5f\n", i, lbmCurrentStep,
+                            area > 0 ? F.x / (0.5f * 0.08f * 0.08f * area) : 0.0f, area, umax);
+                std::fflush(stdout);
+            }
+            if (i == dump_step) ok = ok && dump_all();
+        }
+        shutdownLBM();
+        return ok ? 0 : 1;
+    }
     for (int i = 0; i < steps; ++i) stepLBMCPU(1);   // one call per step, as the frame loop does
     bool ok = dump(dir, "rho", lbmRho) && dump(dir, "ux", lbmUx) && dump(dir, "uy", lbmUy) && dump(dir, "uz", lbmUz) &&
               dump(dir, "tke", lbmTKEField) && dump(dir, "strain", lbmStrainMag) && dump(dir, "ux_world", lbmUxWorld) &&

@@ -197,15 +197,46 @@ def voxels() -> dict:
     return out
 
 
+def box_run() -> dict:
+    """The reference solver on the measured ellipsoid case itself: 256x128x104, TRT +
+    Smagorinsky, U0 0.08, convective outlet, the C2 body's mask (the host voxelizer's, whose
+    digest equals the reference's own vote above).  One call per step for 1000 calls: the
+    drag coefficient, the step counter and max |u| every 50 calls, and SHA-256 digests of
+    the fields after 300 calls."""
+    import hashlib
+    import numpy as np
+    sys.path.insert(0, str(ROOT / "aeros-engine_amd"))
+    import aeros_lbm as A
+    import workloads as W
+    cfg = W.c2_config(A.voxelize)
+    nz, ny, nx = cfg["voxels"].shape
+    d = ROOT / "oracle" / "_ref" / "fields" / "c2_box"
+    d.mkdir(parents=True, exist_ok=True)
+    (d / "mask.u8").write_bytes(np.ascontiguousarray(cfg["voxels"], dtype=np.uint8).tobytes())
+    calls, dump_at = 1000, 300
+    text = run("ref_fields", "box_les", str(d), str(nx), str(ny), str(nz), repr(cfg["tau"]), str(calls), str(dump_at))
+    traj = [{"call": int(c), "step": int(s), "cd": float(cd), "area": float(a), "umax": float(u)}
+            for c, s, cd, a, u in re.findall(r"call (\d+) step (\d+) Cd (\S+) A (\S+) max\|u\| (\S+)", text)]
+    rec = {"grid": [nx, ny, nz], "tau": cfg["tau"], "calls": calls, "dump_after_calls": dump_at,
+           "cv_box": [0.45, 0.40, 0.40], "trajectory": traj, "digests": {}, "samples": {}}
+    for f in sorted(d.glob("*.f32")):
+        raw = f.read_bytes()
+        v = np.frombuffer(raw, dtype="<f4")
+        rec["digests"][f.stem] = hashlib.sha256(raw).hexdigest()
+        idx = [0, v.size // 3, v.size // 2, v.size - 1]
+        rec["samples"][f.stem] = {str(i): float(v[i]) for i in idx}
+    return rec
+
+
 def main() -> None:
     if not (REF / "bench").exists():
         raise SystemExit("oracle/_ref is not built: make -f oracle/ref.mk -j8")
-    which = sys.argv[1:] or ["bench", "physref", "thermal", "validate", "lbm", "fields", "voxels"]
+    which = sys.argv[1:] or ["bench", "physref", "thermal", "validate", "lbm", "fields", "voxels", "box_run"]
     funcs = {"bench": bench, "physref": physref, "thermal": thermal, "validate": validate, "lbm": lbm_suite,
-             "fields": fields, "voxels": voxels}
+             "fields": fields, "voxels": voxels, "box_run": box_run}
     for name in which:
         data = funcs[name]()
-        data["_source"] = f"oracle/_ref/{ {'fields': 'ref_fields', 'voxels': 'ref_voxels'}.get(name, name)} built by oracle/ref.mk"
+        data["_source"] = f"oracle/_ref/{ {'fields': 'ref_fields', 'voxels': 'ref_voxels', 'box_run': 'ref_fields'}.get(name, name)} built by oracle/ref.mk"
         (OUT / f"ref_{name}.json").write_text(json.dumps(data, indent=1, ensure_ascii=False) + "\n")
         print(f"wrote ref_{name}.json", flush=True)
 

@@ -117,3 +117,39 @@ def host_checksums(L):
         s_rho += a
         s_u += b
     return s_rho, s_u
+
+
+# ------------------------------------------------- the measured ellipsoid case
+def follow_box_run(L, ref, calls):
+    """Steps L one call per step (the reference's frame loop) for `calls` calls and
+    checks it against the reference solver's run of the same case (the box_run
+    fixture, printed by oracle/ref_fields.cpp): every field after
+    ref["dump_after_calls"] calls by SHA-256, and at each recorded call the step
+    counter (calls broken by a reset do not advance it), the projected area, the
+    drag coefficient Fx / (0.5 U0^2 A) and max |u| over the lattice, each computed
+    in float as the reference's harness prints them (6 and 5 decimals)."""
+    import hashlib
+    f32 = np.float32
+    area = L.projected_area(0)
+    traj = {t["call"]: t for t in ref["trajectory"]}
+    for call in range(1, calls + 1):
+        L.step(1)
+        if call == ref["dump_after_calls"]:
+            for name, digest in ref["digests"].items():
+                v = np.ascontiguousarray(L.field(name), dtype="<f4").ravel()
+                for i, want in ref["samples"][name].items():
+                    assert float(v[int(i)]) == want, (call, name, i, float(v[int(i)]), want)
+                assert hashlib.sha256(v.tobytes()).hexdigest() == digest, (call, name)
+        t = traj.get(call)
+        if t is None:
+            continue
+        fx = L.cv_force(*ref["cv_box"])[0]
+        cd = float(f32(fx) / (f32(0.5) * f32(0.08) * f32(0.08) * f32(area)))
+        ux, uy, uz = (L.field(k) for k in ("ux", "uy", "uz"))
+        umax = float(np.sqrt(ux * ux + uy * uy + uz * uz).max())
+        step = int(L.scalars()["step"])
+        print(f"call {call} step {step} Cd {cd:.6f} (reference {t['cd']:.6f}) max|u| {umax:.5f}", flush=True)
+        assert step == t["step"], (call, step, t["step"])
+        assert area == t["area"]
+        assert abs(cd - t["cd"]) <= 1e-6 + 1e-6 * abs(t["cd"]), (call, cd, t["cd"])
+        assert abs(umax - t["umax"]) <= 6e-6, (call, umax, t["umax"])

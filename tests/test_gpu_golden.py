@@ -163,3 +163,19 @@ def test_per_cell_fields_match_the_reference_solver_gpu(case):
         for i, want in ref["samples"][name].items():
             assert float(v[int(i)]) == want, (name, i, float(v[int(i)]), want)
         assert hashlib.sha256(v.tobytes()).hexdigest() == digest, name
+
+
+def test_measured_ellipsoid_case_follows_the_reference_solver_gpu():
+    """The measured 256x128x104 ellipsoid case (TRT + Smagorinsky, convective
+    outlet), one call per step for 1000 calls against the reference solver's own
+    run: every field after 300 calls bit for bit, then the drag coefficient, the
+    step counter and max |u| every 50 calls, through the divergence that starts
+    on the side faces near the inlet after about 400 steps and the NaN-storm reset
+    that ends it."""
+    from measures import follow_box_run
+    ref = gold("box_run")
+    print("box run: 256x128x104 ellipsoid, 1000 single-step calls", flush=True)
+    L = A.Lattice(A.default_config(**W.c2_config(A.voxelize)))
+    assert list(L.dims) == ref["grid"] and L.scalars()["tau"] == np.float32(ref["tau"])
+    follow_box_run(L, ref, ref["calls"])
+    L.close()

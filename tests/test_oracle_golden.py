@@ -324,3 +324,16 @@ def test_per_cell_fields_match_the_reference_solver(oracle, case):
         for i, want in ref["samples"][name].items():
             assert float(v[int(i)]) == want, (name, i, float(v[int(i)]), want)
         assert hashlib.sha256(v.tobytes()).hexdigest() == digest, name
+
+
+def test_measured_ellipsoid_case_matches_the_reference_solver(oracle):
+    """The measured 256x128x104 ellipsoid case in the oracle: every field after
+    300 single-step calls bit for bit against the reference solver's dump, and the
+    drag coefficient, step counter and max |u| every 50 calls up to there."""
+    import aeros_lbm as A
+    from measures import follow_box_run
+    ref = json.loads((GOLD / "ref_box_run.json").read_text())
+    L = oracle.OracleLattice(A.default_config(**W.c2_config(A.voxelize)))
+    assert list(L.dims) == ref["grid"]
+    follow_box_run(L, ref, ref["dump_after_calls"])
+    L.close()
