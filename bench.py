@@ -146,15 +146,18 @@ def c3_nusselt(A, W, steps=300000, chunk=20000, precisions=("fp64", "fp32", "fp1
     return out
 
 
-def c4_precision_study(A, W, steps=6000, window=2000, every=100):
-    """C4's body, grid and Re in FP64, FP32, FP16 and BF16 storage from the same start:
+def precision_study(A, W, scale=2.0, re=None, steps=6000, window=2000, every=100):
+    """One body, grid and Re in FP64, FP32, FP16 and BF16 storage from the same start:
     the drag coefficient (control volume 0.45/0.40/0.40 of the domain, A = projected
     solid cells along x) averaged over samples every `every` steps in the last `window`
     steps, and the velocity field's relative L2 over the fluid cells after `steps`, each
-    against the FP64 run.  With the reference's open side faces this flow diverges after
-    about 450 steps in every precision (the reference solver's own run of the 256x128x104
-    case does the same), so the study closes the y and z faces with free-slip walls."""
-    cfg0 = W.c2_config(A.voxelize_gpu, scale=2.0)
+    against the FP64 run.  With the reference's open side faces the ellipsoid flows
+    diverge after about 450 steps in every precision (the reference solver's own run of
+    the 256x128x104 case does the same), so the study closes the y and z faces with
+    free-slip walls.  scale 2, Re 1000 is the 33.5M-cell case; scale 1, Re 100 a steady
+    companion whose drag settles, so its errors are those of the storage and not of an
+    unsteady wake."""
+    cfg0 = W.c2_config(A.voxelize_gpu, scale=scale, re=re)
     cfg0["free_slip"] = W.SIDE_WALLS
     fluid = (np.asarray(cfg0["voxels"]) == 0).reshape(-1)
     out, u64, cd64 = {}, None, None
@@ -170,6 +173,7 @@ def c4_precision_study(A, W, steps=6000, window=2000, every=100):
             cds.append(L.cv_force(0.45, 0.40, 0.40)[0] / (0.5 * 0.08 * 0.08 * area))
         u = np.stack([L.field(k).reshape(-1)[fluid].astype(np.float64) for k in ("ux", "uy", "uz")])
         sc = L.scalars()
+        dims = L.dims
         L.close()
         cd = float(np.mean(cds))
         rec = {"cd_mean": round(cd, 5), "cd_last": round(cds[-1], 5), "steps": sc["step"], "resets": sc["resets"],
@@ -180,10 +184,10 @@ def c4_precision_study(A, W, steps=6000, window=2000, every=100):
             rec["cd_rel_err_vs_fp64"] = float("%.4g" % ((cd - cd64) / cd64))
             rec["u_rel_l2_vs_fp64"] = float("%.4g" % np.sqrt(((u - u64) ** 2).sum() / (u64 ** 2).sum()))
         out[prec] = rec
-        log(f"[c4 precision] {prec}: {rec}")
-    out["note"] = (f"512x256x256, ellipsoid 60/24/24, Re 1000 (tau 0.51152), TRT+LES, convective outlet, "
-                   f"free-slip y/z faces; Cd mean of {window // every} samples over steps "
-                   f"{steps - window + every}..{steps}; L2 after {steps} steps")
+        log(f"[precision {dims[0]}x{dims[1]}x{dims[2]}] {prec}: {rec}")
+    out["note"] = (f"{dims[0]}x{dims[1]}x{dims[2]}, ellipsoid {30 * scale:g}/{12 * scale:g}/{12 * scale:g}, "
+                   f"tau {cfg0['tau']:.5f}, TRT+LES, convective outlet, free-slip y/z faces; Cd mean of "
+                   f"{window // every} samples over steps {steps - window + every}..{steps}; L2 after {steps} steps")
     return out
 
 
@@ -498,7 +502,9 @@ def main():
                                   "free-slip side walls",
                             "c3": "128x64x128 coupled D3Q19+D3Q7 FP32, Rayleigh-Benard Ra 2500 Pr 0.71"}
         sec["host_driver"] = host_driver_leg(A, c2)
-        sec["c4_precision_study"] = c4_precision_study(A, W)
+        sec["c4_precision_study"] = precision_study(A, W)
+        sec["c2_re100_precision_study"] = precision_study(A, W, scale=1.0, re=100.0, steps=16000, window=4000,
+                                                          every=250)
         sec["c3_nusselt"] = c3_nusselt(A, W)
         out["secondary"] = sec
     if not args.no_served:
