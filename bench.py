@@ -29,8 +29,9 @@ the FP64 sphere, the coupled thermal lattice), the 33.5M-cell precision study
 (Cd and velocity L2 of FP32/FP16/BF16 against FP64) and the Rayleigh-Benard
 Nusselt number of FP32/FP16 against FP64 at 300k steps; `served` the served-mode
 stream (solves/s, p50/p99 latency, against one-at-a-time solves);
-`cpu_baseline` times the CPU restatement on the host cores for a bounded
-number of steps of the same case.
+`cpu_baseline` times the reference's own CPU solver (built from its sources
+into oracle/_ref/ref_box) on the host cores for 320 steps of the same case,
+with the CPU restatement's rate beside it (`port`).
 """
 from __future__ import annotations
 
@@ -40,6 +41,7 @@ import ctypes as C
 import glob
 import json
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -218,6 +220,32 @@ def cpu_baseline(cfg, budget_s=12.0, max_steps=40):
     return dict(value=round(cells * n / dt / 1e6, 3), unit="MLUPS", cores=threads, kind="port",
                 sample=f"{n} steps of the same {L.dims[0]}x{L.dims[1]}x{L.dims[2]} case after 1 warm-up step, "
                        f"oracle (C++ restatement, OpenMP {threads} threads)")
+
+
+def cpu_reference(cfg, threads, steps=320):
+    """The reference solver itself on the same case: its own lbm.cpp built from its
+    sources with its bench's flags (-O3, OpenMP) by oracle/ref.mk into oracle/_ref/ref_box,
+    fed the same mask and tau; one warm-up step, then one call of `steps` steps.  None
+    when that binary was not built (the reference sources were absent at build time)."""
+    exe = ROOT / "oracle" / "_ref" / "ref_box"
+    if not exe.exists():
+        return None
+    vox = np.ascontiguousarray(cfg["voxels"], dtype=np.uint8)
+    nz, ny, nx = vox.shape
+    with tempfile.TemporaryDirectory() as d:
+        vox.tofile(os.path.join(d, "mask.u8"))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        r = subprocess.run([str(exe), "box_time", d, str(nx), str(ny), str(nz), repr(cfg["tau"]), str(steps)],
+                           env=env, capture_output=True, text=True, timeout=600)
+    m = re.search(r"box_time (\d+)x(\d+)x(\d+) steps (\d+) seconds (\S+) threads (\d+)", r.stdout)
+    if r.returncode != 0 or not m:
+        log(f"[cpu] reference run failed ({r.returncode}): {r.stderr[-400:]}")
+        return None
+    cells = int(m.group(1)) * int(m.group(2)) * int(m.group(3))
+    n, sec, used = int(m.group(4)), float(m.group(5)), int(m.group(6))
+    return dict(value=round(cells * n / sec / 1e6, 3), unit="MLUPS", cores=used, kind="reference",
+                sample=f"{n} steps (one call, after 1 warm-up step) of the same {nx}x{ny}x{nz} case by the "
+                       f"reference's own CPU solver (stepLBMCPU, -O3, OpenMP {used} threads)")
 
 
 def cpu_others(A, W, budget_s=6.0):
@@ -527,7 +555,14 @@ def main():
             log(f"[traffic] not measured: {err}")
             out["roofline"]["traffic_note"] = err
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(c2)
+        port = cpu_baseline(c2)
+        ref = cpu_reference(c2, port["cores"])
+        if ref:
+            log(f"[cpu] reference solver: {ref['value']} MLUPS on {ref['cores']} threads; restatement {port['value']}")
+            ref["port"] = {k: port[k] for k in ("value", "cores", "sample")}
+            out["cpu_baseline"] = ref
+        else:
+            out["cpu_baseline"] = port
         if not args.no_secondary:
             out["cpu_baseline"]["others"] = cpu_others(A, W)
     print(json.dumps(out), flush=True)

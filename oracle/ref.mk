@@ -24,7 +24,7 @@ CORE    := $(SRC)/lbm.cpp $(SRC)/thermal_lbm.cpp $(SRC)/flow_field.cpp $(SRC)/vo
            $(SRC)/atmosphere.cpp $(SRC)/globals.cpp $(SRC)/debug_log.cpp
 COMMON  := $(SRC)/globals.cpp $(SRC)/debug_log.cpp
 
-BINS := bench bench_fma lbm physref thermal validate voxel ref_fields ref_voxels
+BINS := bench bench_fma lbm physref thermal validate voxel ref_fields ref_box ref_voxels
 .PHONY: all $(BINS)
 all: $(BINS)
 
@@ -58,6 +58,12 @@ voxel: | $(OUT)
 ref_fields: | $(OUT)
 	$(CXX) $(FLAGS) -Dmain=aeros_bench_main_unused -c $(T)/bench_lbm.cpp -o $(OUT)/bench_stubs.o
 	$(CXX) $(FLAGS) $(HERE)ref_fields.cpp $(OUT)/bench_stubs.o $(CORE) -o $(OUT)/ref_fields -pthread
+
+# The same harness built with the reference bench's flags (-O3), for timing
+# the reference solver on the measured case (bench.py's CPU baseline).
+ref_box: | $(OUT)
+	$(CXX) -std=c++17 -O3 -fopenmp -w $(INC) -Dmain=aeros_bench_main_unused -c $(T)/bench_lbm.cpp -o $(OUT)/bench_stubs_o3.o
+	$(CXX) -std=c++17 -O3 -fopenmp -w $(INC) $(HERE)ref_fields.cpp $(OUT)/bench_stubs_o3.o $(CORE) -o $(OUT)/ref_box -pthread
 
 # The reference's voxel vote over a given grid (oracle/ref_voxels.cpp, test
 # infrastructure): header-only BVH, no link stubs needed.

@@ -19,13 +19,23 @@
 //   U0 0.08 along +x, convective outlet; prints the control-volume drag
 //   coefficient (box 0.45/0.40/0.40) every 50 steps and dumps the fields after
 //   <dump-step> steps.
+//
+//   oracle/_ref/ref_box box_time <out-dir> <nx> <ny> <nz> <tau> <steps>
+//   the same case timed: one warm-up step, then one call of <steps> steps;
+//   prints the wall time and the OpenMP thread count.  ref_box is this file
+//   built with -O3 (the reference bench's flags) for timing.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <chrono>
 #include <cstring>
 #include <string>
 #include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 #include "atmosphere.h"
 #include "globals.h"
@@ -157,7 +167,16 @@ int main(int argc, char** argv) {
     }
     const std::string which = argv[1], dir = argv[2];
     int steps = 0, dump_step = -1;
-    if (which == "box_les") {
+    const bool timing = which == "box_time";
+    if (timing) {
+        if (argc < 8) { std::fprintf(stderr, "box_time needs nx ny nz tau steps\n"); return 2; }
+        if (!box_case(dir, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), (float)std::atof(argv[6]))) {
+            std::fprintf(stderr, "cannot read %s/mask.u8\n", dir.c_str());
+            return 1;
+        }
+        steps = std::atoi(argv[7]);
+    }
+    else if (which == "box_les") {
         if (argc < 9) { std::fprintf(stderr, "box_les needs nx ny nz tau steps dump-step\n"); return 2; }
         if (!box_case(dir, std::atoi(argv[3]), std::atoi(argv[4]), std::atoi(argv[5]), (float)std::atof(argv[6]))) {
             std::fprintf(stderr, "cannot read %s/mask.u8\n", dir.c_str());
@@ -186,6 +205,20 @@ int main(int argc, char** argv) {
         if (ok && thermalParams.enabled) ok = dump(dir, "T", thermalT) && dump(dir, "buoyancy", thermalBuoyancyY);
         return ok;
     };
+    if (timing) {   // box_time: one warm-up step, then one call of `steps` steps
+        stepLBMCPU(1);
+        const auto t0 = std::chrono::steady_clock::now();
+        stepLBMCPU(steps);
+        const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        int threads = 1;
+#ifdef _OPENMP
+        threads = omp_get_max_threads();
+#endif
+        std::printf("box_time %dx%dx%d steps %d seconds %.6f threads %d step %d\n", lbmNx, lbmNy, lbmNz, steps, sec,
+                    threads, lbmCurrentStep);
+        shutdownLBM();
+        return 0;
+    }
     if (dump_step >= 0) {   // box_les: drag along the way, fields at dump_step
         const float area = lbmProjectedAreaCells(0);
         bool ok = true;
