@@ -450,6 +450,8 @@ def main():
     ap.add_argument("--no-secondary", action="store_true", help="skip the other precisions and cases")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--no-served", action="store_true", help="skip the served-mode leg")
+    ap.add_argument("--no-studies", action="store_true",
+                    help="skip the host-driver leg and the precision and Nusselt studies of the secondary section")
     ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
@@ -529,11 +531,12 @@ def main():
                             "c1": "96x48x48 FP64 BGK sphere D=12, U0 0.05, zero-gradient outflow, "
                                   "free-slip side walls",
                             "c3": "128x64x128 coupled D3Q19+D3Q7 FP32, Rayleigh-Benard Ra 2500 Pr 0.71"}
-        sec["host_driver"] = host_driver_leg(A, c2)
-        sec["c4_precision_study"] = precision_study(A, W)
-        sec["c2_re100_precision_study"] = precision_study(A, W, scale=1.0, re=100.0, steps=16000, window=4000,
-                                                          every=250)
-        sec["c3_nusselt"] = c3_nusselt(A, W)
+        if not args.no_studies:
+            sec["host_driver"] = host_driver_leg(A, c2)
+            sec["c4_precision_study"] = precision_study(A, W)
+            sec["c2_re100_precision_study"] = precision_study(A, W, scale=1.0, re=100.0, steps=16000, window=4000,
+                                                              every=250)
+            sec["c3_nusselt"] = c3_nusselt(A, W)
         out["secondary"] = sec
     if not args.no_served:
         out["served"] = served_leg(A, W)
