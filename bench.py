@@ -385,11 +385,16 @@ def device_copy(nbytes, reps):
         raise RuntimeError(f"copy probe failed ({rc})")
 
 
-def pmc_child(steps):
-    """What each counter pass runs: the headline case for a few steps, then the copy."""
+PMC_CASES = {"c2": dict(scale=1.0, precision="fp32"), "c4_fp32": dict(scale=2.0, precision="fp32"),
+             "c4_fp16": dict(scale=2.0, precision="fp16"), "c4_bf16": dict(scale=2.0, precision="bf16")}
+
+
+def pmc_child(steps, case="c2"):
+    """What each counter pass runs: one case (the headline one unless named) for a few
+    steps, then the copy."""
     import aeros_lbm as A
     import workloads as W
-    L = A.Lattice(A.default_config(**W.c2_config(A.voxelize)))
+    L = A.Lattice(A.default_config(**W.c2_config(A.voxelize, **PMC_CASES[case])))
     L.step(steps)
     L.close()
     device_copy(COPY_BYTES, 4)
@@ -453,9 +458,10 @@ def main():
     ap.add_argument("--no-studies", action="store_true",
                     help="skip the host-driver leg and the precision and Nusselt studies of the secondary section")
     ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-case", default="c2", choices=sorted(PMC_CASES), help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
-        pmc_child(args.pmc_child)
+        pmc_child(args.pmc_child, args.pmc_case)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}); "
