@@ -130,7 +130,7 @@ struct aeros_lbm {
     std::unique_ptr<EngineBase> eng;
     hipStream_t stream = nullptr;
     int device = 0;
-    std::mutex mu;                          // one call (or frame of the frame worker) at a time
+    mutable std::mutex mu;                  // one call (or frame of the frame worker) at a time
     std::unique_ptr<FrameDriver> frames;    // asynchronous frames, when started
     ~aeros_lbm();
 };
@@ -337,7 +337,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             std::vector<float> Th;
             thermal_initial_field(s, Th);
             for (int b = 0; b < 2; ++b) if ((rc = fill<R>(buoy[b], n, R(0)))) return rc;
-            if ((rc = upload_T(Th.data(), true))) return rc;
+            if ((rc = upload_T(Th.data()))) return rc;
         }
         if ((rc = fill<int64_t>(acc, (int64_t)kAccSets * kAccSet, 0))) return rc;
         Status z{};
@@ -352,16 +352,16 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         return AEROS_OK;
     }
 
-    // T of the current parity (both parities at init, so solid cells match);
-    // populations to g_eq(T, 0) (aeros/src/thermal_lbm.cpp:188-197)
-    int upload_T(const float* Th, bool both) {
+    // T into both parities: the step never writes T of a solid cell, which
+    // keeps the uploaded value as the reference's single thermalT array does;
+    // populations of the current parity to g_eq(T, 0)
+    // (aeros/src/thermal_lbm.cpp:154-197)
+    int upload_T(const float* Th) {
         std::vector<R> tr((size_t)n);
         for (int64_t i = 0; i < n; ++i) tr[(size_t)i] = (R)Th[i];
         const int p = cur & 1;
-        for (int b = 0; b < 2; ++b) {
-            if (!both && b != p) continue;
+        for (int b = 0; b < 2; ++b)
             HIP_TRY(hipMemcpyAsync(T[b].p, tr.data(), (size_t)n * sizeof(R), hipMemcpyHostToDevice, st));
-        }
         hipLaunchKernelGGL((k_thermal_eq<R>), dim3(grid_for(n)), dim3(256), 0, st, g[p].as<R>(), T[p].as<R>(), n);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(st));
@@ -370,7 +370,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
 
     int thermal_upload(const float* Th) override {
         if (!s.thermal_ready) return set_err(AEROS_ERR_STATE, "thermal module not initialised");
-        int rc = upload_T(Th, false);
+        int rc = upload_T(Th);
         if (rc) return rc;
         return thermal_diag();
     }
@@ -877,8 +877,21 @@ int build_handle(aeros_lbm* h) {
     if (rc) return set_err(rc, "%s", msg);
     h->eng.reset(make_engine(h));
     if (!h->eng) return set_err(AEROS_ERR_UNSUPPORTED, "precision %d not built", h->s.cfg.precision);
-    return h->eng->init_lattice();
+    rc = h->eng->init_lattice();
+    if (rc) h->eng.reset();
+    return rc;
 }
+
+}  // namespace
+
+// Serialises a call with the handle's other calls and its frame worker; a
+// handle whose last reset or set-mask failed holds no lattice and refuses
+// every call but those two.
+#define AEROS_LOCK_HANDLE(h)                               \
+    std::lock_guard<std::mutex> lock_((h)->mu);            \
+    if (!(h)->eng) return set_err(AEROS_ERR_STATE, "the handle holds no lattice (its last reset or set-mask failed)")
+
+namespace {
 
 }  // namespace
 
@@ -946,11 +959,12 @@ int aeros_lbm_reset(aeros_lbm* h) {
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
     h->eng.reset();
-    return build_handle(h);
+    return build_handle(h);   // on failure the handle keeps no lattice and refuses other calls
 }
 
 int aeros_lbm_dims(const aeros_lbm* h, int32_t dims[3], float box_min[3], float cell_size[3]) {
-    if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
+    if (!h) return set_err(AEROS_ERR_ARG, "null handle");
+    AEROS_LOCK_HANDLE(h);
     if (dims) { dims[0] = h->s.nx; dims[1] = h->s.ny; dims[2] = h->s.nz; }
     for (int i = 0; i < 3; ++i) {
         if (box_min) box_min[i] = h->s.min[i];
@@ -960,27 +974,36 @@ int aeros_lbm_dims(const aeros_lbm* h, int32_t dims[3], float box_min[3], float 
 }
 
 int aeros_lbm_set_mask(aeros_lbm* h, const uint8_t* solid, int32_t nx, int32_t ny, int32_t nz) {
-    if (!h || !h->eng || !solid) return set_err(AEROS_ERR_ARG, "null argument");
+    if (!h || !solid) return set_err(AEROS_ERR_ARG, "null argument");
+    std::lock_guard<std::mutex> lock_(h->mu);
     if (nx != h->s.nx || ny != h->s.ny || nz != h->s.nz)
         return set_err(AEROS_ERR_ARG, "mask %dx%dx%d does not match the lattice %dx%dx%d", nx, ny, nz, h->s.nx, h->s.ny, h->s.nz);
-    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
-    Setup& s = h->s;
-    for (int64_t i = 0; i < s.total; ++i) { s.solid[(size_t)i] = solid[i] ? 1 : 0; s.ground[(size_t)i] = 0; }
-    s.cfg = h->user_cfg;
+    // The new setup is built aside and replaces the handle's only when it is
+    // complete: the mask, then the ground slab on top of it as initLBM
+    // applies it after the mask transfer (aeros/src/lbm.cpp:596-643), then
+    // wall distance, fluid count and tau.
+    Setup ns = h->s;
+    ns.cfg = h->user_cfg;
+    for (int64_t i = 0; i < ns.total; ++i) { ns.solid[(size_t)i] = solid[i] ? 1 : 0; ns.ground[(size_t)i] = 0; }
+    apply_ground(ns);
     char msg[256] = {0};
-    int rc = setup_after_mask(s, msg, sizeof msg);
+    int rc = setup_after_mask(ns, msg, sizeof msg);
     if (rc) return set_err(rc, "%s", msg);
+    h->eng.reset();
+    h->s = std::move(ns);
     h->eng.reset(make_engine(h));
     if (!h->eng) return set_err(AEROS_ERR_UNSUPPORTED, "precision not built");
-    return h->eng->init_lattice();
+    rc = h->eng->init_lattice();
+    if (rc) h->eng.reset();   // no half-built lattice: the handle refuses calls until a reset or set-mask succeeds
+    return rc;
 }
 
 int aeros_lbm_step(aeros_lbm* h, int32_t steps) {
-    if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
+    if (!h) return set_err(AEROS_ERR_ARG, "null handle");
     if (steps < 0) return set_err(AEROS_ERR_ARG, "negative step count");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    AEROS_LOCK_HANDLE(h);
     (void)hipSetDevice(h->device);
     return h->eng->step(steps);
 }
@@ -990,20 +1013,22 @@ int32_t aeros_lbm_update_steps(float dt, int32_t force_steps, int32_t steps_per_
 }
 
 int aeros_lbm_update(aeros_lbm* h, float dt, int32_t force_steps, int32_t steps_per_frame) {
-    if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h) return set_err(AEROS_ERR_ARG, "null handle");
+    AEROS_LOCK_HANDLE(h);
     (void)hipSetDevice(h->device);
     return run_update(h, update_steps(dt, force_steps, steps_per_frame));
 }
 
 int aeros_lbm_lattice_dt(aeros_lbm* h, float* out) {
-    if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    if (!h || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    AEROS_LOCK_HANDLE(h);
     *out = lattice_dt_world(h->s);
     return AEROS_OK;
 }
 
 int aeros_lbm_frames_start(aeros_lbm* h, uint32_t field_mask) {
-    if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
+    if (!h) return set_err(AEROS_ERR_ARG, "null handle");
+    AEROS_LOCK_HANDLE(h);   // the worker starts waiting for this lock only after start() returns
     if (h->frames) return set_err(AEROS_ERR_STATE, "frames are already running");
     if (field_mask & ~kFrameFields) return set_err(AEROS_ERR_ARG, "field mask 0x%x names a field not held on the device", field_mask);
     if ((field_mask & ((1u << AEROS_FIELD_T) | (1u << AEROS_FIELD_BUOYANCY))) && !h->s.thermal_ready)
@@ -1042,22 +1067,22 @@ int aeros_lbm_frames_stop(aeros_lbm* h) {
 }
 
 int aeros_lbm_read_field(aeros_lbm* h, int32_t field, float* out) {
-    if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    AEROS_LOCK_HANDLE(h);
     (void)hipSetDevice(h->device);
     return h->eng->read_field(field, nullptr, out);
 }
 
 int aeros_lbm_read_field_f64(aeros_lbm* h, int32_t field, double* out) {
-    if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    AEROS_LOCK_HANDLE(h);
     (void)hipSetDevice(h->device);
     return h->eng->read_field(field, out, nullptr);
 }
 
 int aeros_lbm_read(aeros_lbm* h, uint32_t fields, float* rho, float* ux, float* uy, float* uz, float* T) {
-    if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h) return set_err(AEROS_ERR_ARG, "null handle");
+    AEROS_LOCK_HANDLE(h);
     (void)hipSetDevice(h->device);
     struct { uint32_t bit; int id; float* p; } v[5] = {{AEROS_READ_RHO, AEROS_FIELD_RHO, rho}, {AEROS_READ_UX, AEROS_FIELD_UX, ux},
                                                      {AEROS_READ_UY, AEROS_FIELD_UY, uy}, {AEROS_READ_UZ, AEROS_FIELD_UZ, uz},
@@ -1071,37 +1096,37 @@ int aeros_lbm_read(aeros_lbm* h, uint32_t fields, float* rho, float* ux, float* 
 }
 
 int aeros_lbm_get_scalars(aeros_lbm* h, aeros_lbm_scalars* out) {
-    if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    AEROS_LOCK_HANDLE(h);
     h->eng->scalars(out);
     return AEROS_OK;
 }
 
 int aeros_lbm_read_populations(aeros_lbm* h, double* out) {
-    if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    AEROS_LOCK_HANDLE(h);
     (void)hipSetDevice(h->device);
     return h->eng->read_populations(out);
 }
 
 int aeros_lbm_cv_force(aeros_lbm* h, float hx, float hy, float hz, double out[3]) {
-    if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    AEROS_LOCK_HANDLE(h);
     (void)hipSetDevice(h->device);
     return h->eng->cv_force(hx, hy, hz, out);
 }
 
 int aeros_lbm_projected_area(aeros_lbm* h, int32_t axis, float* out) {
-    if (!h || !h->eng || !out) return set_err(AEROS_ERR_ARG, "null argument");
+    if (!h || !out) return set_err(AEROS_ERR_ARG, "null argument");
     if (axis < 0 || axis > 2) return set_err(AEROS_ERR_ARG, "axis %d", axis);
-    std::lock_guard<std::mutex> lock_(h->mu);
+    AEROS_LOCK_HANDLE(h);
     (void)hipSetDevice(h->device);
     return h->eng->projected_area(axis, out);
 }
 
 int aeros_lbm_checksums(aeros_lbm* h, double* rho_sum, double* u_sum) {
-    if (!h || !h->eng || !rho_sum || !u_sum) return set_err(AEROS_ERR_ARG, "null argument");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h || !rho_sum || !u_sum) return set_err(AEROS_ERR_ARG, "null argument");
+    AEROS_LOCK_HANDLE(h);
     (void)hipSetDevice(h->device);
     return h->eng->checksums(rho_sum, u_sum);
 }
@@ -1112,18 +1137,18 @@ int aeros_lbm_rho_sum(aeros_lbm* h, double* rho_sum) {
 }
 
 int aeros_lbm_thermal_upload(aeros_lbm* h, const float* T) {
-    if (!h || !h->eng || !T) return set_err(AEROS_ERR_ARG, "null argument");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h || !T) return set_err(AEROS_ERR_ARG, "null argument");
+    AEROS_LOCK_HANDLE(h);
     (void)hipSetDevice(h->device);
     return h->eng->thermal_upload(T);
 }
 
 int aeros_lbm_thermal_reset(aeros_lbm* h) {
-    if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
+    if (!h) return set_err(AEROS_ERR_ARG, "null handle");
+    AEROS_LOCK_HANDLE(h);
     if (!h->s.thermal_ready) return set_err(AEROS_ERR_STATE, "thermal module not initialised");
     std::vector<float> T;
     thermal_initial_field(h->s, T);
-    std::lock_guard<std::mutex> lock_(h->mu);
     (void)hipSetDevice(h->device);
     return h->eng->thermal_upload(T.data());
 }
@@ -1187,22 +1212,22 @@ int aeros_voxelize_gpu(const float* tris, int64_t ntris, int32_t nx, int32_t ny,
 }
 
 int aeros_lbm_vorticity_q(aeros_lbm* h) {
-    if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h) return set_err(AEROS_ERR_ARG, "null handle");
+    AEROS_LOCK_HANDLE(h);
     HIP_TRY(hipSetDevice(h->device));
     return h->eng->vorticity_q();
 }
 
 int aeros_lbm_sample(aeros_lbm* h, int32_t field, const float* pos, int64_t npos, float* out, uint8_t* valid) {
-    if (!h || !h->eng || npos < 0 || (npos > 0 && (!pos || !out))) return set_err(AEROS_ERR_ARG, "bad sample arguments");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h || npos < 0 || (npos > 0 && (!pos || !out))) return set_err(AEROS_ERR_ARG, "bad sample arguments");
+    AEROS_LOCK_HANDLE(h);
     HIP_TRY(hipSetDevice(h->device));
     return h->eng->sample(field, pos, npos, out, valid);
 }
 
 int aeros_lbm_sample_velocity(aeros_lbm* h, int32_t world, const float* pos, int64_t npos, float* out) {
-    if (!h || !h->eng || npos < 0 || (npos > 0 && (!pos || !out))) return set_err(AEROS_ERR_ARG, "bad sample arguments");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h || npos < 0 || (npos > 0 && (!pos || !out))) return set_err(AEROS_ERR_ARG, "bad sample arguments");
+    AEROS_LOCK_HANDLE(h);
     HIP_TRY(hipSetDevice(h->device));
     return h->eng->sample_velocity(world, pos, npos, out);
 }
@@ -1371,8 +1396,8 @@ int aeros_probe(aeros_backend_info* out) {
 }
 
 int aeros_lbm_kernel_timing(aeros_lbm* h, int32_t enable, double* total_ms, int64_t* launches) {
-    if (!h || !h->eng) return set_err(AEROS_ERR_ARG, "null handle");
-    std::lock_guard<std::mutex> lock_(h->mu);
+    if (!h) return set_err(AEROS_ERR_ARG, "null handle");
+    AEROS_LOCK_HANDLE(h);
     EngineBase* e = h->eng.get();
     if (enable < 0) { e->timed_ms = 0.0; e->timed_launches = 0; e->timing = 0; }
     else e->timing = enable > 2 ? 2 : enable;

@@ -64,6 +64,9 @@ struct Setup {
 int check_boundaries(const aeros_lbm_config& cfg, char* msg, int msglen);
 // initLBM's host half: returns AEROS_OK or an error code; msg gets the reason.
 int setup_domain(const aeros_lbm_config& cfg, Setup& s, char* msg, int msglen);
+// The ground slab of cfg (use_ground / ground_effect) on top of the solid mask:
+// non-solid cells up to the ground plane become ground (s.ground).
+void apply_ground(Setup& s);
 // Everything after the mask is known: wall distance, fluid count, tau, thermal.
 int setup_after_mask(Setup& s, char* msg, int msglen);
 void config_defaults(aeros_lbm_config* c);

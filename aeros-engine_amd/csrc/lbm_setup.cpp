@@ -214,20 +214,24 @@ int setup_domain(const aeros_lbm_config& cfg, Setup& s, char* msg, int msglen) {
             }
         }
     }
-    if (c.ground_effect || c.use_ground) {   // ground slab (aeros/src/lbm.cpp:624-643)
-        float gyw = c.vox_min[1] + c.ground_effect_height;
-        if (c.use_ground) gyw = c.vox_min[1] + c.ground_height;
-        if (!std::isfinite(gyw)) gyw = c.vox_min[1];
-        int gy0 = (int)((gyw - s.min[1]) / s.cell[1]);
-        gy0 = std::min(std::max(gy0, 0), s.ny - 1);
-        for (int z = 0; z < s.nz; ++z)
-            for (int y = 0; y <= gy0; ++y)
-                for (int x = 0; x < s.nx; ++x) {
-                    const int64_t cc = ((int64_t)z * s.ny + y) * s.nx + x;
-                    if (!s.solid[cc]) s.ground[cc] = 1;
-                }
-    }
+    apply_ground(s);
     return AEROS_OK;
+}
+
+void apply_ground(Setup& s) {
+    const aeros_lbm_config& c = s.cfg;
+    if (!(c.ground_effect || c.use_ground)) return;   // ground slab (aeros/src/lbm.cpp:624-643)
+    float gyw = c.vox_min[1] + c.ground_effect_height;
+    if (c.use_ground) gyw = c.vox_min[1] + c.ground_height;
+    if (!std::isfinite(gyw)) gyw = c.vox_min[1];
+    int gy0 = (int)((gyw - s.min[1]) / s.cell[1]);
+    gy0 = std::min(std::max(gy0, 0), s.ny - 1);
+    for (int z = 0; z < s.nz; ++z)
+        for (int y = 0; y <= gy0; ++y)
+            for (int x = 0; x < s.nx; ++x) {
+                const int64_t cc = ((int64_t)z * s.ny + y) * s.nx + x;
+                if (!s.solid[cc]) s.ground[cc] = 1;
+            }
 }
 
 int setup_after_mask(Setup& s, char* msg, int msglen) {

@@ -191,14 +191,22 @@ def ellipsoid_mesh(semi=(30.0, 12.0, 12.0), centre=(0.0, 0.0, 0.0), n_u=316, n_v
     return verts[tri].reshape(-1, 9).astype(f32)
 
 
+# Re of the 256x128x104 case whose divergence and reset the reference solver's own
+# run records (tests/golden/ref_box_run.json): tau 0.5192
+REF_DIVERGING_RE = 300.0
+
+
 def c2_config(voxelize_fn, nx=256, ny=128, nz=104, scale=1.0, seed=12345, re=None,
               precision="fp32", with_mask=True) -> dict:
     """256x128x104, TRT (magic 3/16) + Smagorinsky Cs 0.1, U0 0.08, no body force, an ellipsoid
     STL (semi-axes 30/12/12 cells, ~200k triangles, vertex jitter 0.05 cells, seeded) voxelized
     by the BVH three-ray vote at cell centres.  tau is stated: 0.5 + 3 U0 D / Re with D = 24 (the
-    minor-axis diameter) and Re = 300 -> 0.5192.  scale=2 scales the body and the x/y extents and
-    gives the 512x256x256 (33.5M-cell) case with Re 1000 -> tau 0.51152 (nz is 256 there)."""
-    if scale != 1.0 and nz == 104:
+    minor-axis diameter) and Re = 100 -> 0.5576, at which the case runs its 1000 steps with the
+    reference's open side faces (at Re 300, tau 0.5192, it diverges after about 450 steps, as the
+    reference solver does on it: REF_DIVERGING_RE).  scale=2 scales the body and the x/y extents and
+    gives the 512x256x256 (33.5M-cell) case with Re 1000 -> tau 0.51152 (nz is 256 there); other
+    scales scale all three extents (0.25: 64x32x26)."""
+    if scale == 2.0 and nz == 104:
         nx, ny, nz = int(nx * scale), int(ny * scale), 256
     else:
         nx, ny, nz = int(nx * scale), int(ny * scale), int(nz * scale)
@@ -209,7 +217,7 @@ def c2_config(voxelize_fn, nx=256, ny=128, nz=104, scale=1.0, seed=12345, re=Non
                auto_viscosity=0, inlet_turbulence=0.0, convective_outlet=1, body_force=(0.0, 0.0, 0.0),
                precision=precision, cell_cap=1 << 31)
     D = 24.0 * scale
-    Re = re if re is not None else (300.0 if scale == 1.0 else 1000.0)
+    Re = re if re is not None else (100.0 if scale == 1.0 else 1000.0)
     cfg["tau"] = float(f32(0.5) + f32(3.0) * f32(0.08) * f32(D) / f32(Re))
     if with_mask:
         tris = ellipsoid_mesh(semi=semi, centre=centre, seed=seed, jitter=0.05)
@@ -304,4 +312,26 @@ def reference_field_cases() -> dict:
         # buoyancy large enough to overflow the populations: the NaN-storm reset
         "rb_storm": (thermal_box(32, 18, 8, walls_y=True, rayleigh=1.0e30, fixed_wall_temps=True,
                                  perturbation=0.02), 12),
+        # tau from the Reynolds number (autoViscosity, aeros/src/lbm.cpp:741-815) in the
+        # standard atmosphere at 1500 m: at 0.0004 m/s tau lands inside [0.51, 1.5], at
+        # the bench's 20 m/s it clamps to 0.51
+        "bench12_auto": (bench_sphere_auto(0.0004), 60),
+        "bench12_auto_clamped": (bench_sphere_auto(20.0), 60),
     }
+
+
+# airTemperature and airDensity of the reference's standard atmosphere
+# (updateAtmosphereParams, aeros/src/atmosphere.cpp:176-191) at sea level and
+# at 1500 m, as float32
+AIR_SEA_LEVEL = (288.149994, 1.2250123)
+AIR_1500M = (278.399994, 1.05807602)
+
+
+def bench_sphere_auto(flow_speed: float) -> dict:
+    """The bench sphere at n = 12 with LES and tau from the Reynolds number
+    (autoViscosity): Sutherland viscosity at the air temperature, Re = rho V L / mu
+    with L = max_dim, then tau = 1/2 + 3 U0 L_LB / Re clamped to [0.51, 1.5]."""
+    cfg = bench_sphere(12, les=True)
+    cfg.update(auto_viscosity=1, flow_speed=float(np.float32(flow_speed)), air_temperature=AIR_1500M[0],
+               air_density=AIR_1500M[1])
+    return cfg

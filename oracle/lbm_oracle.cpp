@@ -313,6 +313,7 @@ public:
     aeros_lbm_config cfg{};
     int storage = AEROS_FP32;     // FP32/FP64 exact, FP16/BF16 quantised (R = float)
     int Nx = 0, Ny = 0, Nz = 0, total = 0;
+    aeros_lbm_config user_cfg{};   // the parameters init was called with (tau and U0 unsanitised)
     float minX = 0, minY = 0, minZ = 0, csX = 0.1f, csY = 0.1f, csZ = 0.1f;
     std::vector<R> f, fNext;
     std::vector<R> rho, ux, uy, uz, tke, strain, uxW, uyW, uzW, pres;
@@ -356,6 +357,7 @@ public:
     // aeros/src/lbm.cpp:405-886
     int init(const aeros_lbm_config& c) {
         cfg = c;
+        user_cfg = c;
         const int32_t* vox = c.voxels;
         const bool haveVox = vox && c.vox_n[0] > 0 && c.vox_n[1] > 0 && c.vox_n[2] > 0;
         const long long capPadded = c.cell_cap > 0 ? c.cell_cap : 8LL * 1024 * 1024;
@@ -451,19 +453,23 @@ public:
                 }
             }
         }
-        // ground slab (aeros/src/lbm.cpp:624-643)
-        if (c.ground_effect || c.use_ground) {
-            float groundY = c.vox_min[1] + c.ground_effect_height;
-            if (c.use_ground) groundY = c.vox_min[1] + c.ground_height;
-            if (!std::isfinite(groundY)) groundY = c.vox_min[1];
-            int gy0 = (int)((groundY - minY) / csY);
-            if (gy0 < 0) gy0 = 0;
-            if (gy0 >= Ny) gy0 = Ny - 1;
-            for (int z = 0; z < Nz; ++z)
-                for (int y = 0; y <= gy0; ++y)
-                    for (int x = 0; x < Nx; ++x) { const int cc = idx(x, y, z); if (!solid[cc]) ground[cc] = 1; }
-        }
+        ground_slab();
         return finish_init();
+    }
+
+    // ground slab over the solid mask (aeros/src/lbm.cpp:624-643)
+    void ground_slab() {
+        const aeros_lbm_config& c = cfg;
+        if (!(c.ground_effect || c.use_ground)) return;
+        float groundY = c.vox_min[1] + c.ground_effect_height;
+        if (c.use_ground) groundY = c.vox_min[1] + c.ground_height;
+        if (!std::isfinite(groundY)) groundY = c.vox_min[1];
+        int gy0 = (int)((groundY - minY) / csY);
+        if (gy0 < 0) gy0 = 0;
+        if (gy0 >= Ny) gy0 = Ny - 1;
+        for (int z = 0; z < Nz; ++z)
+            for (int y = 0; y <= gy0; ++y)
+                for (int x = 0; x < Nx; ++x) { const int cc = idx(x, y, z); if (!solid[cc]) ground[cc] = 1; }
     }
 
     // Everything after the mask transfer: wall distance, fluid count, the
@@ -589,9 +595,12 @@ public:
         return AEROS_OK;
     }
 
+    // A new solid mask on the same grid: the ground slab goes back on top of
+    // it, as initLBM applies it after every mask transfer.
     int set_mask(const uint8_t* m) {
+        cfg = user_cfg;   // initLBM starts again from the caller's parameters
         for (int cc = 0; cc < total; ++cc) { solid[cc] = m[cc] ? 1 : 0; ground[cc] = 0; }
-        cfg.tau = cfg.tau;  // tau was written back by init; user tau path keeps it
+        ground_slab();
         return finish_init();
     }
 
@@ -1334,6 +1343,7 @@ float ora_last_divergence(void* p, int* step) {
     return ORA_DISPATCH(h, (*step = S.lastDivergedStep, S.lastDivergedRho));
 }
 
+float ora_sutherland(float t_kelvin) { return sutherland(t_kelvin); }
 float ora_q_half(float x) { return q_half(x); }
 float ora_q_bf16(float x) { return q_bf16(x); }
 

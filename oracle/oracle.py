@@ -55,6 +55,7 @@ def load() -> C.CDLL:
         "ora_thermal_reset": (C.c_int, [vp]),
         "ora_equilibrium_moments": (C.c_int, [C.c_float] * 4 + [P(C.c_float)] * 4),
         "ora_q_half": (C.c_float, [C.c_float]),
+        "ora_sutherland": (C.c_float, [C.c_float]),
         "ora_q_bf16": (C.c_float, [C.c_float]),
         "ora_voxelize": (C.c_int, [P(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_int32,
                                    P(C.c_float), P(C.c_float), P(C.c_int32)]),
@@ -185,6 +186,11 @@ def equilibrium_moments(rho, ux, uy, uz):
     o = [C.c_float() for _ in range(4)]
     ok = lib.ora_equilibrium_moments(rho, ux, uy, uz, *[C.byref(v) for v in o])
     return bool(ok), tuple(v.value for v in o)
+
+
+def sutherland(t_kelvin: float) -> float:
+    """Dynamic viscosity of air (Pa s) at t_kelvin, airSutherlandViscosity."""
+    return load().ora_sutherland(t_kelvin)
 
 
 def q_half(x: float) -> float:

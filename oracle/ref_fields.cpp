@@ -12,6 +12,14 @@
 //
 //   oracle/_ref/ref_fields <case> <out-dir>
 //   cases: bench12_les  bench12_les_thermal  bench16  rb_walls  rb_storm
+//          bench12_auto  bench12_auto_clamped (tau from Re, autoViscosity on;
+//          the standard atmosphere at 1500 m; the first at a flow speed whose
+//          Reynolds number puts tau inside [0.51, 1.5], the second at the
+//          bench's 20 m/s, where tau clamps to 0.51).  After init these print
+//          the scalars initLBM derived (tau, viscosity, Re_phys, Re_LB and
+//          the air state) to 9 significant digits.
+//   oracle/_ref/ref_fields atmosphere -   the air temperature and density of
+//          the reference's standard atmosphere at 0 and 1500 m.
 //
 //   oracle/_ref/ref_fields box_les <out-dir> <nx> <ny> <nz> <tau> <steps> <dump-step>
 //   a lattice-aligned box (padding 0) with the solid mask read from
@@ -166,6 +174,14 @@ int main(int argc, char** argv) {
         return 2;
     }
     const std::string which = argv[1], dir = argv[2];
+    if (which == "atmosphere") {   // the air state updateAtmosphereParams derives at a few altitudes
+        for (float alt : {0.0f, 1500.0f}) {
+            altitude = alt;
+            updateAtmosphereParams();
+            std::printf("altitude %.9g air_temperature %.9g air_density %.9g\n", altitude, airTemperature, airDensity);
+        }
+        return 0;
+    }
     int steps = 0, dump_step = -1;
     const bool timing = which == "box_time";
     if (timing) {
@@ -194,10 +210,22 @@ int main(int argc, char** argv) {
         thermalParams.rayleigh = 1.0e30f;
         steps = 12;
     }
+    else if (which == "bench12_auto" || which == "bench12_auto_clamped") {
+        sphere_case(12, true, false);
+        lbmParams.autoViscosity = true;
+        altitude = 1500.0f;
+        updateAtmosphereParams();
+        if (which == "bench12_auto") flowSpeed = 0.0004f;
+        steps = 60;
+    }
     else { std::fprintf(stderr, "unknown case %s\n", which.c_str()); return 2; }
     shutdownLBM();
     initLBM();
     if (!lbmInitialized) { std::fprintf(stderr, "init failed\n"); return 1; }
+    if (lbmParams.autoViscosity)
+        std::printf("auto tau %.9g viscosity %.9g re_phys %.9g re_lb %.9g air_temperature %.9g air_density %.9g "
+                    "flow_speed %.9g\n", lbmParams.tau, lbmParams.viscosity, lbmReynolds, lbmReynoldsLB,
+                    airTemperature, airDensity, flowSpeed);
     auto dump_all = [&]() {
         bool ok = dump(dir, "rho", lbmRho) && dump(dir, "ux", lbmUx) && dump(dir, "uy", lbmUy) && dump(dir, "uz", lbmUz) &&
                   dump(dir, "tke", lbmTKEField) && dump(dir, "strain", lbmStrainMag) && dump(dir, "ux_world", lbmUxWorld) &&

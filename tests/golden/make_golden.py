@@ -147,7 +147,8 @@ def lbm_suite() -> dict:
 # Per-cell field dumps of oracle/_ref/ref_fields (the reference solver run on
 # small cases): recorded as SHA-256 digests of the raw float32 bytes, plus a
 # few values for a readable failure.
-FIELD_CASES = ["bench12_les", "bench12_les_thermal", "bench16", "rb_walls", "rb_storm"]
+FIELD_CASES = ["bench12_les", "bench12_les_thermal", "bench16", "rb_walls", "rb_storm", "bench12_auto",
+               "bench12_auto_clamped"]
 
 
 def fields() -> dict:
@@ -161,6 +162,10 @@ def fields() -> dict:
         m = re.search(r"(\d+)x(\d+)x(\d+) steps (\d+) step (\d+)", text)
         rec = {"grid": [int(g) for g in m.groups()[:3]], "steps": int(m.group(4)), "calls": "one per step",
                "final_step": int(m.group(5)), "digests": {}, "samples": {}}
+        a = re.search(r"auto (.*)", text)
+        if a:   # tau from Re: what initLBM derived, and the air state it used
+            kv = a.group(1).split()
+            rec["init"] = {kv[i]: float(kv[i + 1]) for i in range(0, len(kv), 2)}
         for f in sorted(d.glob("*.f32")):
             raw = f.read_bytes()
             v = np.frombuffer(raw, dtype="<f4")
@@ -168,6 +173,9 @@ def fields() -> dict:
             idx = [0, v.size // 3, v.size // 2, v.size - 1]
             rec["samples"][f.stem] = {str(i): float(v[i]) for i in idx}
         out["cases"][case] = rec
+    out["atmosphere"] = [{"altitude": float(a), "air_temperature": float(t), "air_density": float(r)}
+                         for a, t, r in re.findall(r"altitude (\S+) air_temperature (\S+) air_density (\S+)",
+                                                   run("ref_fields", "atmosphere", "-"))]
     return out
 
 
@@ -208,7 +216,7 @@ def box_run() -> dict:
     sys.path.insert(0, str(ROOT / "aeros-engine_amd"))
     import aeros_lbm as A
     import workloads as W
-    cfg = W.c2_config(A.voxelize)
+    cfg = W.c2_config(A.voxelize, re=W.REF_DIVERGING_RE)
     nz, ny, nx = cfg["voxels"].shape
     d = ROOT / "oracle" / "_ref" / "fields" / "c2_box"
     d.mkdir(parents=True, exist_ok=True)

@@ -158,6 +158,10 @@ def test_per_cell_fields_match_the_reference_solver_gpu(case):
     for _ in range(steps):                 # one call per step, as the reference's frame loop
         L.step(1)
     assert L.scalars()["step"] == ref["final_step"]   # calls broken by a reset do not advance it
+    for key, name in (("tau", "tau"), ("viscosity", "viscosity"), ("re_phys", "reynolds_phys"),
+                      ("re_lb", "reynolds_lb")):
+        if "init" in ref:   # tau from Re: the scalars the reference's initLBM derived
+            assert np.float32(L.scalars()[name]) == np.float32(ref["init"][key]), key
     for name, digest in ref["digests"].items():
         v = np.ascontiguousarray(L.field(name), dtype="<f4").ravel()
         for i, want in ref["samples"][name].items():
@@ -175,7 +179,7 @@ def test_measured_ellipsoid_case_follows_the_reference_solver_gpu():
     from measures import follow_box_run
     ref = gold("box_run")
     print("box run: 256x128x104 ellipsoid, 1000 single-step calls", flush=True)
-    L = A.Lattice(A.default_config(**W.c2_config(A.voxelize)))
+    L = A.Lattice(A.default_config(**W.c2_config(A.voxelize, re=W.REF_DIVERGING_RE)))
     assert list(L.dims) == ref["grid"] and L.scalars()["tau"] == np.float32(ref["tau"])
     follow_box_run(L, ref, ref["calls"])
     L.close()

@@ -88,30 +88,94 @@ def test_fp64_against_fp64_oracle(oracle):
     assert abs(rg - ro) <= 1e-12 * abs(ro)
 
 
+SIXTEEN_BIT_CASES = {
+    # name: (case builder, steps of the long run, fields compared, RMS bound of the long run)
+    "bench_sphere_bgk": (lambda: W.bench_sphere(12), 40, ("rho", "ux", "uy", "uz"), 1.25),
+    # the 33.5M-cell scheme at 64x32x26: TRT (s- != s+), Smagorinsky with Van Driest
+    # damping, convective outlet, the jittered ellipsoid STL scaled by 1/4
+    "ellipsoid_trt_les_outlet": (lambda: W.c2_config(A.voxelize, scale=0.25, re=50.0), 80,
+                                 ("rho", "ux", "uy", "uz", "strain", "tke"), 1.25),
+    "bench_sphere_les_thermal": (lambda: W.bench_sphere(12, les=True, thermal=True), 40,
+                                 ("rho", "ux", "uy", "uz", "T", "buoyancy"), 1.25),
+    # a Guo force whose per-step increment 16-bit storage resolves (at the Poiseuille
+    # case's 7.8e-6 it lies below half a half-precision ulp of the stored deviations,
+    # so both sides round it away and compare only their roundings).  The flow is
+    # one-dimensional: every x-z plane holds the same values, so each rounding tie that
+    # falls the other way is repeated over a whole plane and the long run's accumulated
+    # rounding does not average out; its bound is 2x.
+    "channel_trt_guo": (lambda: W.channel(18, 0.65, 1e-4, trt=True), 400, ("rho", "ux"), 2.0),
+}
+
+# D3Q19 weights as the kernels hold them (float32): stored values are f - w
+W19 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, dtype=np.float32).astype(np.float64)
+
+
+def ulp16(d, prec):
+    """Spacing of the 16-bit format at the stored deviation d (fp16: 10 mantissa
+    bits, subnormal below 2^-14; bf16: 7 mantissa bits)."""
+    a = np.abs(d)
+    e = np.floor(np.log2(np.where(a > 0, a, 1.0)))
+    if prec == "fp16":
+        e = np.maximum(e, -14.0)
+    return 2.0 ** (e - (10 if prec == "fp16" else 7))
+
+
 @pytest.mark.parametrize("prec", ["fp16", "bf16"])
-def test_16bit_storage_against_quantised_and_fp64_oracle(oracle, prec):
-    """16-bit deviation storage has no reference counterpart; its collision is
-    the reference's relaxation on deviations, folded into fused multiply-adds.
-    Against the FP64 oracle its RMS error must stay within 1.25x that of the
-    oracle's own 16-bit quantisation (reference arithmetic, same storage), and
+@pytest.mark.parametrize("name", sorted(SIXTEEN_BIT_CASES))
+def test_16bit_stored_populations_match_the_quantised_oracle(oracle, prec, name):
+    """The 16-bit collision is the reference's relaxation folded into fused
+    multiply-adds (with the hardware reciprocal and square root), so its FP32
+    results differ from the oracle's reference-order arithmetic by about 1e-9;
+    stored to 16 bits they are the same number unless they straddle a rounding
+    tie.  After the first step every stored population is the oracle's; after the
+    second, at least 99.5 % are and the others lie one unit in the last place
+    away (the second step is the first with the Smagorinsky stencil and the
+    relaxation of a non-equilibrium state)."""
+    build, _, _, _ = SIXTEEN_BIT_CASES[name]
+    case = dict(build(), precision=prec)
+    G, O = make_pair(oracle, **case)
+    fluid = (G.field("solid") < 0.5).reshape(-1)
+    for k in (1, 2):
+        G.step(1)
+        O.step(1)
+        dg = G.populations().reshape(-1, 19)[fluid] - W19
+        dq = O.populations().reshape(-1, 19)[fluid] - W19
+        diff = np.abs(dg - dq)
+        same = float((diff == 0).mean())
+        print(f"{name} {prec} step {k}: {same:.4%} of stored populations identical", flush=True)
+        if k == 1:
+            assert same == 1.0, same
+        else:
+            assert same >= 0.995, same
+            assert np.all(diff <= ulp16(dq, prec)), float((diff / ulp16(dq, prec)).max())
+
+
+@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("name", sorted(SIXTEEN_BIT_CASES))
+def test_16bit_storage_against_quantised_and_fp64_oracle(oracle, prec, name):
+    """Over a long run the GPU's 16-bit storage error against the FP64 oracle stays
+    that of the oracle's own 16-bit quantisation (reference arithmetic, same storage):
+    its RMS within the case's bound (1.25x, 2x for the one-dimensional channel) and
     its largest error (one cell, noisier) within 2.5x."""
-    case = W.bench_sphere(12)
-    G, O = make_pair(oracle, **case, precision=prec)
-    O64 = oracle.OracleLattice(A.default_config(**case, precision="fp64"))
+    build, steps, names, bound = SIXTEEN_BIT_CASES[name]
+    case = dict(build(), precision=prec)
+    G, O = make_pair(oracle, **case)
+    O64 = oracle.OracleLattice(A.default_config(**dict(case, precision="fp64")))
     for L in (G, O, O64):
-        L.step(40)
-    for name in ("rho", "ux", "uy", "uz"):
-        g = G.field(name, dtype=np.float64)
-        q = O.field(name, dtype=np.float64)
-        e = O64.field(name, dtype=np.float64)
+        L.step(steps)
+    assert G.scalars()["step"] == O64.scalars()["step"] == steps
+    for field in names:
+        g = G.field(field, dtype=np.float64)
+        q = O.field(field, dtype=np.float64)
+        e = O64.field(field, dtype=np.float64)
         rms_gpu = float(np.sqrt(np.mean((g - e) ** 2)))
         rms_q = float(np.sqrt(np.mean((q - e) ** 2)))
         max_gpu = float(np.abs(g - e).max())
         max_q = float(np.abs(q - e).max())
-        print(f"{prec} {name}: rms |gpu-fp64| {rms_gpu:.3e} |quantised oracle-fp64| {rms_q:.3e}; "
+        print(f"{name} {prec} {field}: rms |gpu-fp64| {rms_gpu:.3e} |quantised oracle-fp64| {rms_q:.3e}; "
               f"max {max_gpu:.3e} vs {max_q:.3e}")
-        assert rms_gpu <= 1.25 * rms_q + 1e-7, (name, rms_gpu, rms_q)
-        assert max_gpu <= 2.5 * max_q + 1e-6, (name, max_gpu, max_q)
+        assert rms_gpu <= bound * rms_q + 1e-7, (field, rms_gpu, rms_q)
+        assert max_gpu <= 2.5 * max_q + 1e-6, (field, max_gpu, max_q)
 
 
 def test_thermal_coupled_bitwise(oracle):

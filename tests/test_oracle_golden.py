@@ -160,6 +160,30 @@ def test_poiseuille(oracle, trt):
         same("%.2e", r["mass_drift"], PHYS["poiseuille_mass_drift"])
 
 
+@pytest.mark.parametrize("tau,trt,steps", [(0.65, False, 40000), (0.65, True, 40000), (1.0, False, 20000)])
+def test_poiseuille_fp64_closed_form(oracle, tau, trt, steps):
+    """The FP64 oracle's steady channel against the closed form of half-way
+    bounce-back with two relaxation rates: u(s) = F/(2 nu) (H^2/4 - s^2 + (16 L - 3)/12)
+    with nu = (tau - 1/2)/3 and L = (tau - 1/2)(1/s- - 1/2), s- = 1/tau for BGK and the
+    reference's TRT rule otherwise (aeros/src/lbm.cpp:1196-1202).  Holds to 2e-7 of
+    u_max at these settings; swapping the two s- rules moves it by 2.4e-5, so the
+    1e-6 bound pins viscosity, Guo forcing, the wall position and the s- rule in FP64,
+    where the reference's own FP32 numbers carry 1e-3 of mass drift."""
+    F, ny = 7.8e-6, 34
+    L = oracle.OracleLattice(oracle.default_config(**W.channel(ny, tau, F, trt=trt), precision="fp64"))
+    L.step(steps)
+    ux = L.field("ux", dtype=np.float64)
+    nu = (tau - 0.5) / 3.0
+    s_minus = 1.0 / max(0.05, 0.1875 / (1.0 / tau - 0.5) + 0.5) if trt else 1.0 / tau
+    lam = (tau - 0.5) * (1.0 / s_minus - 0.5)
+    s = np.arange(1, ny - 1) - 0.5 * (ny - 1)
+    exact = F / (2.0 * nu) * ((ny - 2) ** 2 / 4.0 - s * s + (16.0 * lam - 3.0) / 12.0)
+    dev = np.abs(ux[:, 1:ny - 1, :] - exact[None, :, None]).max() / exact.max()
+    assert dev < 1e-6, dev
+    r = channel_profile(L, F, tau, ny)
+    assert r["mass_drift"] < 1e-12, r
+
+
 @pytest.mark.slow
 def test_grid_convergence_and_viscosity(oracle):
     ls = []
@@ -178,6 +202,42 @@ def test_grid_convergence_and_viscosity(oracle):
     same("%.3e", lam, PHYS["decay_lambda"])
     nu = (0.65 - 0.5) / 3.0
     assert abs(lam - nu * math.pi ** 2 / 32 ** 2) / (nu * math.pi ** 2 / 32 ** 2) < 0.03
+
+
+# ---------------------------------------------------------------- air viscosity and Re (lbm suite)
+LBM = gold("lbm")
+
+
+def test_sutherland_viscosity(oracle):
+    """airSutherlandViscosity (aeros/src/flow_field.h:46-54) against the lbm suite's
+    printed values (%.4e), the closed form mu0 (T/T0)^1.5 (T0 + S)/(T + S) in double,
+    and the guards the suite checks (below 50 K and NaN fall back to 288.15 K)."""
+    for t, key in ((273.15, "mu_273"), (288.15, "mu_288")):
+        mu = oracle.sutherland(t)
+        same("%.4e", mu, LBM[key])
+        closed = 1.716e-5 * (t / 273.15) ** 1.5 * (273.15 + 110.4) / (t + 110.4)
+        assert abs(mu - closed) <= 2e-7 * closed, (t, mu, closed)
+    assert oracle.sutherland(15.0) == oracle.sutherland(288.15)
+    assert math.isfinite(oracle.sutherland(float("nan")))
+    assert oracle.sutherland(300.0) > oracle.sutherland(288.15) > oracle.sutherland(273.15)
+
+
+def test_air_states_are_the_references():
+    """The air states the tau-from-Re cases feed in are the reference atmosphere's."""
+    got = {a["altitude"]: (a["air_temperature"], a["air_density"]) for a in gold("fields")["atmosphere"]}
+    assert np.float32(got[0.0]).tolist() == np.float32(W.AIR_SEA_LEVEL).tolist()
+    assert np.float32(got[1500.0]).tolist() == np.float32(W.AIR_1500M).tolist()
+
+
+def test_reynolds_number_of_the_suite_sphere(oracle):
+    """initLBM's Re = rho V L / mu(T) for the suite's sphere in the sea-level
+    atmosphere the suite sets up (%.0f as it prints it)."""
+    t, rho = W.AIR_SEA_LEVEL
+    L = oracle.OracleLattice(oracle.default_config(**W.lbm_suite_sphere(), air_temperature=t, air_density=rho))
+    sc = L.scalars()
+    same("%.0f", sc["reynolds_phys"], LBM["re_phys"])
+    closed = rho * 20.0 * 1.4 / oracle.sutherland(t)
+    assert abs(sc["reynolds_phys"] - closed) <= 1e-6 * closed
 
 
 # ---------------------------------------------------------------- thermal (G6)
@@ -319,11 +379,24 @@ def test_per_cell_fields_match_the_reference_solver(oracle, case):
     for _ in range(steps):                 # one call per step, as the reference's frame loop
         L.step(1)
     assert L.scalars()["step"] == ref["final_step"]   # calls broken by a reset do not advance it
+    check_init_scalars(L.scalars(), ref)
     for name, digest in ref["digests"].items():
         v = np.ascontiguousarray(L.field(name), dtype="<f4").ravel()
         for i, want in ref["samples"][name].items():
             assert float(v[int(i)]) == want, (name, i, float(v[int(i)]), want)
         assert hashlib.sha256(v.tobytes()).hexdigest() == digest, name
+
+
+def check_init_scalars(sc, ref):
+    """tau, viscosity and both Reynolds numbers of a tau-from-Re case, as the
+    reference's initLBM derived them (printed to 9 significant digits, so the
+    float32 round trip is exact)."""
+    init = ref.get("init")
+    if not init:
+        return
+    for key, name in (("tau", "tau"), ("viscosity", "viscosity"), ("re_phys", "reynolds_phys"),
+                      ("re_lb", "reynolds_lb")):
+        assert np.float32(sc[name]) == np.float32(init[key]), (key, sc[name], init[key])
 
 
 def test_measured_ellipsoid_case_matches_the_reference_solver(oracle):
@@ -333,7 +406,7 @@ def test_measured_ellipsoid_case_matches_the_reference_solver(oracle):
     import aeros_lbm as A
     from measures import follow_box_run
     ref = json.loads((GOLD / "ref_box_run.json").read_text())
-    L = oracle.OracleLattice(A.default_config(**W.c2_config(A.voxelize)))
+    L = oracle.OracleLattice(A.default_config(**W.c2_config(A.voxelize, re=W.REF_DIVERGING_RE)))
     assert list(L.dims) == ref["grid"]
     follow_box_run(L, ref, ref["dump_after_calls"])
     L.close()
