@@ -1,19 +1,18 @@
 #!/usr/bin/env python3
-"""Throughput of the MI355X LBM hot path (MLUPS and %HBM) on the 3.41M-cell case.
+"""Throughput of the MI355X LBM hot path (MLUPS and %HBM) on the 33.5M-cell case.
 
 Workload (one step = one D3Q19 collide+stream pass over the whole lattice):
-256x128x104 cells, FP32, TRT (magic 3/16) + Smagorinsky Cs 0.1, U0 0.08 along
-+x, convective outlet, no body force, a synthetic ellipsoid STL (semi-axes
-30/12/12 cells, 199,080 triangles, vertex jitter U(-0.05, 0.05) cells from a
-seeded generator) voxelized by the BVH three-ray vote at the cell centres.
-The lattice and its mask are resident in HBM before the timed region.
+512x256x256 cells, FP32, TRT (magic 3/16) + Smagorinsky Cs 0.1, U0 0.08 along
++x, convective outlet, no body force, tau 0.51152 (Re 1000 on the minor-axis
+diameter, 48 cells), a synthetic ellipsoid STL (semi-axes 60/24/24 cells,
+199,080 triangles, vertex jitter U(-0.05, 0.05) cells from a seeded generator)
+voxelized by the BVH three-ray vote at the cell centres.  The lattice and its
+mask are resident in HBM before the timed region.
 
 Each timed run warms up, restarts the lattice from its initial state and times
-steps 1..K of the case.  The case diverges after about 450 steps (as the
-reference solver does on it: open side faces, tau 0.5192) and its step loop
-breaks after 980 steps; the launches after the break return at once, so
-throughput and kernel time count only the steps the call advanced
-(`steps_done`).
+steps 1..K of the case; throughput and kernel time count only the steps the call
+advanced (`steps_done`: the reference's step loop breaks on a NaN storm or a
+diverged mean density, and the launches after a break return at once).
 
 Prints one JSON line.  `value` is MLUPS = cells * steps_done / seconds / 1e6
 over the timed call; `roofline` uses the step kernel's average duration from one
@@ -24,14 +23,14 @@ the Smagorinsky stencil + 1 flag byte).  `roofline.traffic` is the HBM bytes
 per step-kernel launch from two rocprofv3 counter passes (FETCH_SIZE, then
 WRITE_SIZE) over a short run of the same case, each scaled by what the same
 counter reads for a plain copy kernel of known size in the same process.
-`secondary` holds the other precisions and cases (33.5M cells FP32/FP16/BF16,
-the FP64 sphere, the coupled thermal lattice), the 33.5M-cell precision study
-(Cd and velocity L2 of FP32/FP16/BF16 against FP64) and the Rayleigh-Benard
-Nusselt number of FP32/FP16 against FP64 at 300k steps; `served` the served-mode
-stream (solves/s, p50/p99 latency, against one-at-a-time solves);
-`cpu_baseline` times the reference's own CPU solver (built from its sources
-into oracle/_ref/ref_box) on the host cores for 320 steps of the same case,
-with the CPU restatement's rate beside it (`port`).
+`cpu_baseline` times the oracle's C++ restatement of the reference solver (the
+reference itself refuses lattices above 20 Mi cells) on the host cores for a
+bounded sample of steps of the same case, with the CPU model; the reference's own
+solver, built from its sources, is timed on the 256x128x104 case beside it.
+`secondary` holds the other precisions and cases (33.5M cells FP16/BF16, the
+256x128x104 case, the FP64 sphere, the coupled thermal lattice) and, unless
+--no-studies, the precision and Nusselt studies; `served` the served-mode stream
+(solves/s, p50/p99 latency, against one-at-a-time solves).
 """
 from __future__ import annotations
 
@@ -201,13 +200,36 @@ def line(r, bpc):
             "bytes_per_cell": bpc, "cells": r["cells"]}
 
 
-def cpu_baseline(cfg, budget_s=12.0, max_steps=40):
+def cpu_model():
+    """The host CPU's model name and the threads this process may use."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count() or 1
+    return model, avail
+
+
+def cpu_baseline(cfg, budget_s=15.0, max_steps=60):
+    """The oracle (C++ restatement of the reference solver, same algorithm and AoS
+    layout, OpenMP) on the host cores: one warm-up step, then single steps until
+    budget_s has passed (at least 3, at most max_steps)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     O.build()
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    threads = O.threads(min(threads, 16))
+    model, avail = cpu_model()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or avail
+    threads = O.threads(min(threads, avail, 16))
+    t0 = time.perf_counter()
     L = O.OracleLattice(O.default_config(**cfg))
+    t_init = time.perf_counter() - t0
     L.step(1)
     n = 0
     t0 = time.perf_counter()
@@ -216,10 +238,13 @@ def cpu_baseline(cfg, budget_s=12.0, max_steps=40):
         n += 1
     dt = time.perf_counter() - t0
     cells = L.ncells
+    dims = L.dims
     L.close()
     return dict(value=round(cells * n / dt / 1e6, 3), unit="MLUPS", cores=threads, kind="port",
-                sample=f"{n} steps of the same {L.dims[0]}x{L.dims[1]}x{L.dims[2]} case after 1 warm-up step, "
-                       f"oracle (C++ restatement, OpenMP {threads} threads)")
+                cpu_model=model, cpus_available=avail, init_s=round(t_init, 1),
+                sample=f"{n} single steps ({dt:.1f} s) of the same {dims[0]}x{dims[1]}x{dims[2]} case after 1 "
+                       f"warm-up step: the oracle (C++ restatement of the reference solver, same algorithm and AoS "
+                       f"layout, OpenMP {threads} threads); the reference itself refuses lattices above 20 Mi cells")
 
 
 def cpu_reference(cfg, threads, steps=320):
@@ -249,14 +274,14 @@ def cpu_reference(cfg, threads, steps=320):
 
 
 def cpu_others(A, W, budget_s=6.0):
-    """The restatement on the other configurations, a few seconds each: C1 in FP64, the
-    coupled thermal C3, and C4 in FP32 (the reference itself stops at 20 Mi cells)."""
+    """The restatement on the other configurations, a few seconds each: C1 in FP64,
+    the 256x128x104 case and the coupled thermal C3 in FP32."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     out = {}
     c3, c3T = W.c3_rayleigh_benard()
-    for name, cfg, T in (("c1_fp64", W.c1_sphere_fp64(), None), ("c3_thermal_fp32", c3, c3T),
-                         ("c4_fp32", W.c2_config(A.voxelize, scale=2.0), None)):
+    for name, cfg, T in (("c1_fp64", W.c1_sphere_fp64(), None), ("c2_fp32", W.c2_config(A.voxelize), None),
+                         ("c3_thermal_fp32", c3, c3T)):
         L = O.OracleLattice(O.default_config(**cfg))
         if T is not None:
             L.thermal_upload(T)
@@ -387,14 +412,15 @@ def device_copy(nbytes, reps):
 
 PMC_CASES = {"c2": dict(scale=1.0, precision="fp32"), "c4_fp32": dict(scale=2.0, precision="fp32"),
              "c4_fp16": dict(scale=2.0, precision="fp16"), "c4_bf16": dict(scale=2.0, precision="bf16")}
+HEADLINE_CASE = "c4_fp32"
 
 
-def pmc_child(steps, case="c2"):
+def pmc_child(steps, case=HEADLINE_CASE):
     """What each counter pass runs: one case (the headline one unless named) for a few
     steps, then the copy."""
     import aeros_lbm as A
     import workloads as W
-    L = A.Lattice(A.default_config(**W.c2_config(A.voxelize, **PMC_CASES[case])))
+    L = A.Lattice(A.default_config(**W.c2_config(A.voxelize_gpu, **PMC_CASES[case])))
     L.step(steps)
     L.close()
     device_copy(COPY_BYTES, 4)
@@ -420,7 +446,7 @@ def read_counter(d, name):
     return sum(step) / len(step), sum(copy) / len(copy)
 
 
-def measure_traffic(steps=20, timeout=240):
+def measure_traffic(steps=20, timeout=240, case=HEADLINE_CASE):
     """HBM bytes per step-kernel launch from FETCH_SIZE and WRITE_SIZE passes."""
     exe = shutil.which("rocprofv3")
     if not exe:
@@ -429,7 +455,7 @@ def measure_traffic(steps=20, timeout=240):
     for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="aeros_pmc_", dir="/tmp")
         cmd = [exe, "--pmc", ctr, "-d", d, "-o", "run", "-f", "csv", "--", sys.executable, str(ROOT / "bench.py"),
-               "--pmc-child", str(steps)]
+               "--pmc-child", str(steps), "--pmc-case", case]
         log(f"[traffic] counter pass {ctr}")
         try:
             r = subprocess.run(cmd, cwd="/tmp", env={**os.environ, "TMPDIR": "/tmp"}, timeout=timeout,
@@ -449,7 +475,7 @@ def measure_traffic(steps=20, timeout=240):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-secondary", action="store_true", help="skip the other precisions and cases")
@@ -458,7 +484,7 @@ def main():
     ap.add_argument("--no-studies", action="store_true",
                     help="skip the host-driver leg and the precision and Nusselt studies of the secondary section")
     ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
-    ap.add_argument("--pmc-case", default="c2", choices=sorted(PMC_CASES), help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-case", default=HEADLINE_CASE, choices=sorted(PMC_CASES), help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.pmc_child:
         pmc_child(args.pmc_child, args.pmc_case)
@@ -476,25 +502,17 @@ def main():
         raise SystemExit(f"no usable gfx950 device: {info['status']}")
     log(f"device: {info['status']}")
 
-    tris = W.ellipsoid_mesh(semi=(30.0, 12.0, 12.0), centre=(256 * 0.3, 64.0, 52.0), seed=12345, jitter=0.05)
-    grid = ((256, 128, 104), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    # the 33.5M-cell case: the x2 ellipsoid's mask by the device vote (bit-identical to the
+    # host vote and to the reference's own, by the voxelizer tests)
     t0 = time.perf_counter()
-    m_host = A.voxelize(tris, *grid)
-    t_host = time.perf_counter() - t0
-    A.voxelize_gpu(tris, *grid)   # first call: device set-up
-    t0 = time.perf_counter()
-    m_gpu = A.voxelize_gpu(tris, *grid)
-    t_gpu = time.perf_counter() - t0
-    if not np.array_equal(m_host, m_gpu):
-        raise RuntimeError("device voxelizer disagrees with the host voxelizer")
-    c2 = W.c2_config(lambda t, d, b, c: m_gpu)   # the same mesh and grid as above
-    log(f"C2 mask voxelized in {t_host:.3f} s on the host ({os.cpu_count()} threads), {t_gpu:.4f} s with the "
-        f"device vote ({int(m_gpu.sum())} solid cells)")
-    r = run_gpu(A, c2, args.steps, args.warmup, "c2 fp32")
+    c4 = W.c2_config(A.voxelize_gpu, scale=2.0)
+    log(f"C4 mask voxelized on the device in {time.perf_counter() - t0:.2f} s (mesh generation included)")
+    r = run_gpu(A, c4, args.steps, args.warmup, "c4 fp32")
     bpc = BYTES_PER_CELL[("fp32", True)]
     achieved = bpc * r["cells"] / (r["kernel_ms"] * 1e-3) / 1e9
+    nx, ny, nz = r["dims"]
     out = {
-        "metric": "MLUPS (D3Q19 TRT+Smagorinsky FP32, 3.41M cells) and %HBM peak",
+        "metric": "MLUPS (D3Q19 TRT+Smagorinsky FP32, 33.5M cells) and %HBM peak",
         "value": round(r["mlups"], 1),
         "unit": "MLUPS",
         "n_gpus": 1,
@@ -508,8 +526,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded jittered ellipsoid STL, voxelized by the BVH three-ray vote)",
-        "config": {"workload": "256x128x104 D3Q19 TRT(3/16)+Smagorinsky Cs0.1 FP32, ellipsoid 30/12/12 cells, "
-                               "U0 0.08, tau 0.5192, convective outlet",
+        "config": {"workload": f"{nx}x{ny}x{nz} D3Q19 TRT(3/16)+Smagorinsky Cs0.1 FP32, ellipsoid 60/24/24 cells "
+                               f"(199,080 triangles), U0 0.08, tau {c4['tau']:.5f} (Re 1000), convective outlet",
                    "cells": r["cells"], "parallelism": "single GPU",
                    "timed": "steps 1..K of the case: W warm-up steps, then a restart from the initial state"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -518,22 +536,40 @@ def main():
                      "timing": "one HIP event pair around the timed launches on the solver stream",
                      "bytes_per_cell": bpc},
         "pct_hbm_from_mlups": round(r["mlups"] * 1e6 * bpc / (HBM_PEAK_GBS * 1e9), 4),
-        "voxelize": {"mesh": "199,080-triangle jittered ellipsoid on 256x128x104",
-                     "host_s": round(t_host, 4), "host_threads": os.cpu_count(), "gpu_s": round(t_gpu, 4),
-                     "note": "BVH build on the host included in both; masks identical"},
     }
+    c2 = None
     if not args.no_secondary:
         sec = {}
-        for prec in ("fp32", "fp16", "bf16"):
-            c4 = W.c2_config(A.voxelize_gpu, scale=2.0, precision=prec)
-            sec[f"c4_{prec}"] = line(run_gpu(A, c4, 200, 20, f"c4 {prec}"), BYTES_PER_CELL[(prec, True)])
-        sec["fp16_over_fp32"] = round(sec["c4_fp16"]["mlups"] / sec["c4_fp32"]["mlups"], 3)
-        sec["bf16_over_fp32"] = round(sec["c4_bf16"]["mlups"] / sec["c4_fp32"]["mlups"], 3)
+        tris = W.ellipsoid_mesh(semi=(30.0, 12.0, 12.0), centre=(256 * 0.3, 64.0, 52.0), seed=12345, jitter=0.05)
+        grid = ((256, 128, 104), (0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+        t0 = time.perf_counter()
+        m_host = A.voxelize(tris, *grid)
+        t_host = time.perf_counter() - t0
+        A.voxelize_gpu(tris, *grid)   # first call: device set-up
+        t0 = time.perf_counter()
+        m_gpu = A.voxelize_gpu(tris, *grid)
+        t_gpu = time.perf_counter() - t0
+        if not np.array_equal(m_host, m_gpu):
+            raise RuntimeError("device voxelizer disagrees with the host voxelizer")
+        c2 = W.c2_config(lambda t, d, b, c: m_gpu)   # the same mesh and grid as above
+        sec["voxelize"] = {"mesh": "199,080-triangle jittered ellipsoid on 256x128x104",
+                           "host_s": round(t_host, 4), "host_threads": os.cpu_count(), "gpu_s": round(t_gpu, 4),
+                           "note": "BVH build on the host included in both; masks identical"}
+        sec["c2_fp32"] = line(run_gpu(A, c2, 1000, 50, "c2 fp32"), BYTES_PER_CELL[("fp32", True)])
+        sec["c2_fp32"]["tau"] = round(c2["tau"], 5)
+        sec["c4_fp32"] = line(r, bpc)
+        for prec in ("fp16", "bf16"):
+            c4p = dict(c4, precision=prec)
+            sec[f"c4_{prec}"] = line(run_gpu(A, c4p, 200, 20, f"c4 {prec}"), BYTES_PER_CELL[(prec, True)])
+        sec["fp16_over_fp32"] = round(sec["c4_fp16"]["mlups"] / r["mlups"], 3)
+        sec["bf16_over_fp32"] = round(sec["c4_bf16"]["mlups"] / r["mlups"], 3)
         sec["c1_fp64"] = line(run_gpu(A, W.c1_sphere_fp64(), 1000, 50, "c1 fp64"), BYTES_PER_CELL[("fp64", False)])
         sec["c1_fp64"].update(c1_results(A, W))
         c3cfg, c3T = W.c3_rayleigh_benard()
         sec["c3_thermal_fp32"] = line(run_gpu(A, c3cfg, 500, 20, "c3 thermal", T=c3T), BYTES_PER_CELL[("thermal", False)])
         sec["workloads"] = {"c4": "512x256x256 (33.5M cells), ellipsoid scaled x2, Re 1000, TRT+LES",
+                            "c2": f"256x128x104 (3.41M cells), ellipsoid 30/12/12, Re 100 (tau {c2['tau']:.5f}), "
+                                  "TRT+LES, 1000 steps",
                             "c1": "96x48x48 FP64 BGK sphere D=12, U0 0.05, zero-gradient outflow, "
                                   "free-slip side walls",
                             "c3": "128x64x128 coupled D3Q19+D3Q7 FP32, Rayleigh-Benard Ra 2500 Pr 0.71"}
@@ -552,6 +588,7 @@ def main():
             total = tr["FETCH_SIZE"]["bytes"] + tr["WRITE_SIZE"]["bytes"]
             out["roofline"]["traffic"] = round(total)
             out["roofline"]["traffic_detail"] = {
+                "case": HEADLINE_CASE,
                 "read_bytes": round(tr["FETCH_SIZE"]["bytes"]), "write_bytes": round(tr["WRITE_SIZE"]["bytes"]),
                 "bytes_per_cell": round(total / r["cells"], 1),
                 "over_algorithmic": round(total / (bpc * r["cells"]), 3),
@@ -564,14 +601,16 @@ def main():
             log(f"[traffic] not measured: {err}")
             out["roofline"]["traffic_note"] = err
     if not args.no_cpu:
-        port = cpu_baseline(c2)
+        port = cpu_baseline(c4)
+        log(f"[cpu] restatement on the headline case: {port['value']} MLUPS on {port['cores']} threads "
+            f"({port['cpu_model']})")
+        out["cpu_baseline"] = port
+        if c2 is None:
+            c2 = W.c2_config(A.voxelize_gpu)
         ref = cpu_reference(c2, port["cores"])
         if ref:
-            log(f"[cpu] reference solver: {ref['value']} MLUPS on {ref['cores']} threads; restatement {port['value']}")
-            ref["port"] = {k: port[k] for k in ("value", "cores", "sample")}
-            out["cpu_baseline"] = ref
-        else:
-            out["cpu_baseline"] = port
+            log(f"[cpu] reference solver on 256x128x104: {ref['value']} MLUPS on {ref['cores']} threads")
+            out["cpu_baseline"]["reference_256x128x104"] = ref
         if not args.no_secondary:
             out["cpu_baseline"]["others"] = cpu_others(A, W)
     print(json.dumps(out), flush=True)
