@@ -304,6 +304,7 @@ def load_library(path: os.PathLike | str | None = None) -> C.CDLL:
         "aeros_stl_detect": (C.c_int, [C.c_char_p, C.c_int64, P(StlInfo)]),
         "aeros_lbm_vorticity_q": (C.c_int, [vp]),
         "aeros_lbm_sample": (C.c_int, [vp, C.c_int32, P(C.c_float), C.c_int64, P(C.c_float), P(C.c_uint8)]),
+        "aeros_lbm_sample_noalloc": (C.c_int, [vp, C.c_int32, P(C.c_float), C.c_int64, P(C.c_float), P(C.c_uint8)]),
         "aeros_lbm_sample_velocity": (C.c_int, [vp, C.c_int32, P(C.c_float), C.c_int64, P(C.c_float)]),
         "aeros_stl_parse": (C.c_int, [C.c_char_p, C.c_int64, P(P(C.c_float)), P(P(C.c_float)), P(C.c_int64),
                                       P(C.c_int64)]),
@@ -569,15 +570,16 @@ class Lattice:
         """computeLBMVorticityAndQ on the device (fields "vorticity", "qcrit", "strain")."""
         _check(self.lib.aeros_lbm_vorticity_q(self.h), "aeros_lbm_vorticity_q")
 
-    def sample(self, name: str, pos):
-        """World-space trilinear sampler of a scalar field at (N, 3) points: (values, valid)."""
+    def sample(self, name: str, pos, noalloc: bool = False):
+        """World-space trilinear sampler of a scalar field at (N, 3) points: (values, valid).
+        noalloc: through aeros_lbm_sample_noalloc (fixed device staging, at most 65536 points)."""
         pos = np.ascontiguousarray(pos, dtype=np.float32).reshape(-1, 3)
         out = np.empty(pos.shape[0], dtype=np.float32)
         valid = np.empty(pos.shape[0], dtype=np.uint8)
         F = C.POINTER(C.c_float)
-        _check(self.lib.aeros_lbm_sample(self.h, FIELD_NAMES[name], pos.ctypes.data_as(F), pos.shape[0],
-                                         out.ctypes.data_as(F), valid.ctypes.data_as(C.POINTER(C.c_uint8))),
-               "aeros_lbm_sample")
+        fn = "aeros_lbm_sample_noalloc" if noalloc else "aeros_lbm_sample"
+        _check(getattr(self.lib, fn)(self.h, FIELD_NAMES[name], pos.ctypes.data_as(F), pos.shape[0],
+                                     out.ctypes.data_as(F), valid.ctypes.data_as(C.POINTER(C.c_uint8))), fn)
         return out, valid.astype(bool)
 
     def sample_velocity(self, pos, world: bool = True):

@@ -109,6 +109,11 @@ struct EngineBase {
     virtual void scalars(aeros_lbm_scalars* o) = 0;
     virtual int vorticity_q() = 0;
     virtual int sample(int field, const float* pos, int64_t np, float* out, uint8_t* valid) = 0;
+    // The same sampler with its device staging (positions, values, flags of np
+    // points) in `stage`; allocates nothing, and refuses a field whose device
+    // buffer would first have to be allocated.
+    virtual int sample_staged(int field, const float* pos, int64_t np, float* out, uint8_t* valid,
+                              unsigned char* stage) = 0;
     virtual int sample_velocity(int world, const float* pos, int64_t np, float* out) = 0;
     // A field, as float, into a device buffer on the solver stream (no wait).
     virtual int field_device(int id, float* dst) = 0;
@@ -704,38 +709,69 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         return G;
     }
 
-    // Batched world-space samplers (aeros/src/lbm.cpp:1845-1999) on the device.
-    int sample(int field, const float* pos, int64_t np, float* out, uint8_t* valid) override {
-        const R* src = nullptr;
-        int mode = 0;
+    // The field and mode of a scalar sampler; with may_alloc false a field whose
+    // device buffer does not exist yet (vorticity, Q) is refused.
+    int sample_source(int field, bool may_alloc, const R** src, int* mode) {
+        *src = nullptr;
+        *mode = 0;
         switch (field) {
-            case AEROS_FIELD_PRESSURE: src = world[3].as<R>(); break;
-            case AEROS_FIELD_RHO: src = fld[0].as<R>(); mode = 1; break;
-            case AEROS_FIELD_TKE: src = fld[4].as<R>(); break;
-            case AEROS_FIELD_STRAIN: src = fld[5].as<R>(); break;
+            case AEROS_FIELD_PRESSURE: *src = world[3].as<R>(); break;
+            case AEROS_FIELD_RHO: *src = fld[0].as<R>(); *mode = 1; break;
+            case AEROS_FIELD_TKE: *src = fld[4].as<R>(); break;
+            case AEROS_FIELD_STRAIN: *src = fld[5].as<R>(); break;
             case AEROS_FIELD_VORTICITY:
             case AEROS_FIELD_QCRIT: {
-                int rc;
-                if (!vort.p && (rc = vorticity_q())) return rc;
-                src = (field == AEROS_FIELD_VORTICITY ? vort : qcrit).as<R>();
+                if (!vort.p) {
+                    if (!may_alloc)
+                        return set_err(AEROS_ERR_STATE, "vorticity and Q are not on the device yet: call "
+                                                        "aeros_lbm_vorticity_q first");
+                    if (int rc = vorticity_q()) return rc;
+                }
+                *src = (field == AEROS_FIELD_VORTICITY ? vort : qcrit).as<R>();
                 break;
             }
             default: return set_err(AEROS_ERR_ARG, "field %d has no world sampler", field);
         }
+        return AEROS_OK;
+    }
+
+    // Positions in, values and flags out through device staging dp / dout / dval.
+    int sample_run(const R* src, int mode, const float* pos, int64_t np, float* out, uint8_t* valid, float* dp,
+                   float* dout, uint8_t* dval) {
+        HIP_TRY(hipMemcpyAsync(dp, pos, (size_t)np * 3 * sizeof(float), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL((k_sample<R>), dim3(grid_for(np)), dim3(256), 0, st, src, mask.as<uint32_t>(), sample_grid(),
+                           dp, np, mode, dout, dval);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, dout, (size_t)np * sizeof(float), hipMemcpyDeviceToHost, st));
+        if (valid) HIP_TRY(hipMemcpyAsync(valid, dval, (size_t)np, hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        return AEROS_OK;
+    }
+
+    // Batched world-space samplers (aeros/src/lbm.cpp:1845-1999) on the device.
+    int sample(int field, const float* pos, int64_t np, float* out, uint8_t* valid) override {
+        const R* src;
+        int mode;
+        if (int rc = sample_source(field, true, &src, &mode)) return rc;
         if (np <= 0) return AEROS_OK;
         DevBuf dp, dout, dval;
         int rc;
         if ((rc = dp.alloc((size_t)np * 3 * sizeof(float))) || (rc = dout.alloc((size_t)np * sizeof(float))) ||
             (rc = dval.alloc((size_t)np)))
             return rc;
-        HIP_TRY(hipMemcpyAsync(dp.p, pos, (size_t)np * 3 * sizeof(float), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL((k_sample<R>), dim3(grid_for(np)), dim3(256), 0, st, src, mask.as<uint32_t>(), sample_grid(),
-                           dp.as<float>(), np, mode, dout.as<float>(), dval.as<uint8_t>());
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)np * sizeof(float), hipMemcpyDeviceToHost, st));
-        if (valid) HIP_TRY(hipMemcpyAsync(valid, dval.p, (size_t)np, hipMemcpyDeviceToHost, st));
-        HIP_TRY(hipStreamSynchronize(st));
-        return AEROS_OK;
+        return sample_run(src, mode, pos, np, out, valid, dp.as<float>(), dout.as<float>(), dval.as<uint8_t>());
+    }
+
+    // stage: 3 np floats of positions, then np floats of values, then np flags
+    int sample_staged(int field, const float* pos, int64_t np, float* out, uint8_t* valid,
+                      unsigned char* stage) override {
+        const R* src;
+        int mode;
+        if (int rc = sample_source(field, false, &src, &mode)) return rc;
+        if (np <= 0) return AEROS_OK;
+        float* dp = reinterpret_cast<float*>(stage);
+        float* dout = dp + 3 * np;
+        return sample_run(src, mode, pos, np, out, valid, dp, dout, reinterpret_cast<uint8_t*>(dout + np));
     }
 
     int sample_velocity(int world_units, const float* pos, int64_t np, float* out) override {
@@ -882,6 +918,44 @@ int build_handle(aeros_lbm* h) {
     return rc;
 }
 
+}  // namespace
+
+// Device staging of aeros_lbm_sample_noalloc, the sampler for callers that
+// cannot allocate: positions (3 floats), value (float) and flag (byte) of up
+// to kNoAllocPoints points.  Each device that runs the library holds its own
+// copy until the process ends.  Host code never passes the array's name to a
+// kernel or a copy (on the host it is only a stand-in for the symbol): it uses
+// the device address hipGetSymbolAddress returns, taken once per device on the
+// first call there with that device current.
+constexpr int64_t kNoAllocPoints = 65536;
+constexpr size_t kNoAllocBytes = (size_t)kNoAllocPoints * (3 * sizeof(float) + sizeof(float) + 1);
+constexpr int kNoAllocDevices = 64;
+static __device__ __align__(256) unsigned char noalloc_scratch[kNoAllocBytes];
+
+namespace {
+std::mutex g_noalloc_mu;                                   // guards the address table
+unsigned char* g_noalloc_ptr[kNoAllocDevices] = {};        // device address of noalloc_scratch per device
+std::mutex g_noalloc_use[kNoAllocDevices];                 // one call at a time per device
+
+// The current device's address of noalloc_scratch (the caller has made `dev` current).
+int noalloc_stage(int dev, unsigned char** out) {
+    if (dev < 0 || dev >= kNoAllocDevices)
+        return set_err(AEROS_ERR_UNSUPPORTED, "device %d is beyond the %d devices of the allocation-free staging",
+                       dev, kNoAllocDevices);
+    std::lock_guard<std::mutex> g(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return set_err(AEROS_ERR_HIP, "hipGetSymbolAddress of the sampler staging failed on device %d: %s", dev,
+                           hipGetErrorString(e));
+        }
+        g_noalloc_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    *out = g_noalloc_ptr[dev];
+    return AEROS_OK;
+}
 }  // namespace
 
 // Serialises a call with the handle's other calls and its frame worker; a
@@ -1223,6 +1297,20 @@ int aeros_lbm_sample(aeros_lbm* h, int32_t field, const float* pos, int64_t npos
     AEROS_LOCK_HANDLE(h);
     HIP_TRY(hipSetDevice(h->device));
     return h->eng->sample(field, pos, npos, out, valid);
+}
+
+int aeros_lbm_sample_noalloc(aeros_lbm* h, int32_t field, const float* pos, int64_t npos, float* out, uint8_t* valid) {
+    if (!h || npos < 0 || (npos > 0 && (!pos || !out))) return set_err(AEROS_ERR_ARG, "bad sample arguments");
+    if (npos > kNoAllocPoints)
+        return set_err(AEROS_ERR_TOO_LARGE, "%lld points exceed the %lld-point staging of the allocation-free sampler",
+                       (long long)npos, (long long)kNoAllocPoints);
+    AEROS_LOCK_HANDLE(h);
+    HIP_TRY(hipSetDevice(h->device));
+    unsigned char* stage = nullptr;
+    if (int rc = noalloc_stage(h->device, &stage)) return rc;
+    // handles on one device share its staging: their calls take turns
+    std::lock_guard<std::mutex> turn(g_noalloc_use[h->device]);
+    return h->eng->sample_staged(field, pos, npos, out, valid, stage);
 }
 
 int aeros_lbm_sample_velocity(aeros_lbm* h, int32_t world, const float* pos, int64_t npos, float* out) {

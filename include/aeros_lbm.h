@@ -389,6 +389,19 @@ int aeros_lbm_vorticity_q(aeros_lbm* h);
  * QCRIT (0 when no fluid corner).  pos holds npos (x, y, z) world points;
  * out npos values; valid (may be NULL) 1 where the sampler found fluid data. */
 int aeros_lbm_sample(aeros_lbm* h, int32_t field, const float* pos, int64_t npos, float* out, uint8_t* valid);
+/* aeros_lbm_sample without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, per-frame probe and streamline
+ * loops that must not call hipMalloc).  No counterpart in the reference.  Same
+ * fields, arguments and results as aeros_lbm_sample; the positions, values and
+ * flags are staged on the device in a fixed library array (1.06 MiB per
+ * device, held until the process ends) instead of per-call buffers.  At most
+ * 65536 points per call (AEROS_ERR_TOO_LARGE beyond).  VORTICITY and QCRIT
+ * need their device buffers to exist: AEROS_ERR_STATE until the first
+ * aeros_lbm_vorticity_q (or aeros_lbm_update that runs it) on the handle.
+ * Calls on one device share the array; the library runs them one at a time,
+ * and the array keeps the last call's contents. */
+int aeros_lbm_sample_noalloc(aeros_lbm* h, int32_t field, const float* pos, int64_t npos, float* out,
+                             uint8_t* valid);
 /* getLBMVelocityLB (world = 0, lattice units) or getLBMVelocityWorld
  * (world = 1: free stream outside the lattice box, 0 in solid cells, else
  * scaled by flowSpeed/U0 and capped at 2 flowSpeed), aeros/src/lbm.cpp:
