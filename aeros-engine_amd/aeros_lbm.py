@@ -22,10 +22,10 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "build" / "libaeros_lbm.so"
 
-FP32, FP64, FP16, BF16 = 0, 1, 2, 3
+FP32, FP64, FP16, BF16, FP32_FP64C = 0, 1, 2, 3, 4
 # Domain faces (Config.free_slip): x = 0, x = Nx-1, y = 0, y = Ny-1, z = 0, z = Nz-1.
 FACE_X0, FACE_X1, FACE_Y0, FACE_Y1, FACE_Z0, FACE_Z1 = 1, 2, 4, 8, 16, 32
-PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp16": FP16, "bf16": BF16}
+PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp16": FP16, "bf16": BF16, "fp32_fp64c": FP32_FP64C}
 
 (FIELD_RHO, FIELD_UX, FIELD_UY, FIELD_UZ, FIELD_UX_WORLD, FIELD_UY_WORLD, FIELD_UZ_WORLD,
  FIELD_PRESSURE, FIELD_TKE, FIELD_STRAIN, FIELD_SOLID, FIELD_GROUND, FIELD_WALLDIST,

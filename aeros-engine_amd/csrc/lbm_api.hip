@@ -876,6 +876,7 @@ EngineBase* make_engine(aeros_lbm* h) {
         case AEROS_FP64: return new (std::nothrow) Engine<double, double, double>(h);
         case AEROS_FP16: return new (std::nothrow) Engine<float, _Float16, _Float16>(h);
         case AEROS_BF16: return new (std::nothrow) Engine<float, bf16_t, _Float16>(h);
+        case AEROS_FP32_FP64C: return new (std::nothrow) Engine<double, float, float>(h);
         default: return nullptr;
     }
 }
@@ -903,7 +904,7 @@ int build_handle(aeros_lbm* h) {
     // bytes (float, half, bf16) must stay below 2^31 / 2^32
     {
         const int prec = h->s.cfg.precision;
-        const int64_t es = prec == AEROS_FP64 ? 8 : (prec == AEROS_FP32 ? 4 : 2);
+        const int64_t es = prec == AEROS_FP64 ? 8 : ((prec == AEROS_FP32 || prec == AEROS_FP32_FP64C) ? 4 : 2);
         const int64_t lim = prec == AEROS_FP64 ? (((int64_t)1 << 31) / Q - 1) : (((int64_t)1 << 32) / (Q * es) - 1);
         if (h->s.total > lim)
             return set_err(AEROS_ERR_TOO_LARGE, "%lld cells exceed the %lld-cell limit of one lattice at this precision",
@@ -983,7 +984,7 @@ aeros_lbm* aeros_lbm_create(const aeros_lbm_config* cfg, int* err) {
     if (!cfg) return fail(set_err(AEROS_ERR_ARG, "null config"));
     if (cfg->struct_size != (int32_t)sizeof(aeros_lbm_config))
         return fail(set_err(AEROS_ERR_ARG, "config struct_size %d != %d", cfg->struct_size, (int)sizeof(aeros_lbm_config)));
-    if (cfg->precision < AEROS_FP32 || cfg->precision > AEROS_BF16)
+    if (cfg->precision < AEROS_FP32 || cfg->precision > AEROS_FP32_FP64C)
         return fail(set_err(AEROS_ERR_ARG, "unknown precision %d", cfg->precision));
     {
         char msg[160];

@@ -77,6 +77,13 @@ template <> struct Pop<float, float> {
     __device__ static float ld(const float* p, uint32_t i, float) { return at(p, i); }
     __device__ static void st(float* p, uint32_t i, float v, float) { st_nt(at(p, i), v); }
 };
+// float deviations f - w with double arithmetic: the stored value's spacing
+// follows the deviation, not w, so the small per-step changes of a slow
+// decay are kept (a float f near w = 1/18 has a spacing of 3.7e-9)
+template <> struct Pop<float, double> {
+    __device__ static double ld(const float* p, uint32_t i, double w) { return w + (double)at(p, i); }
+    __device__ static void st(float* p, uint32_t i, double v, double w) { st_nt(at(p, i), (float)(v - w)); }
+};
 template <> struct Pop<double, double> {
     __device__ static double ld(const double* p, uint32_t i, double) { return at(p, i); }
     __device__ static void st(double* p, uint32_t i, double v, double) { st_nt(at(p, i), v); }

@@ -46,6 +46,9 @@ extern "C" {
 #define AEROS_FP64 1   /* double populations, double collision                           */
 #define AEROS_FP16 2   /* IEEE half deviations f_i - w_i, float collision                */
 #define AEROS_BF16 3   /* bfloat16 deviations f_i - w_i, float collision                 */
+#define AEROS_FP32_FP64C 4   /* float deviations f_i - w_i and velocity stencil, double
+                                collision and thermal lattice: FP32's memory traffic, FP64's
+                                arithmetic (no counterpart in the reference)               */
 
 /* field identifiers for aeros_lbm_read_field */
 #define AEROS_FIELD_RHO        0   /* lbmRho        (aeros/src/lbm.h:132)          */
@@ -128,7 +131,7 @@ typedef struct aeros_lbm_config {
     float   cold_temp;
     float   perturbation;
     /* MI355X side */
-    int32_t precision;            /* AEROS_FP32 / FP64 / FP16 / BF16                       */
+    int32_t precision;            /* AEROS_FP32 / FP64 / FP16 / BF16 / FP32_FP64C          */
     int64_t cell_cap;             /* 0 = the reference's caps (8 Mi padded, 20 Mi hard);
                                      >0 = explicit cap on Nx*Ny*Nz                          */
     int32_t device;               /* HIP device ordinal                                    */

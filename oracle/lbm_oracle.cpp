@@ -311,7 +311,8 @@ struct Bvh {
 template <class R> class Sim {
 public:
     aeros_lbm_config cfg{};
-    int storage = AEROS_FP32;     // FP32/FP64 exact, FP16/BF16 quantised (R = float)
+    int storage = AEROS_FP32;     // FP32/FP64 exact, FP16/BF16 quantised (R = float), FP32_FP64C
+                                  // deviations f - w and the velocity stencil rounded to float (R = double)
     int Nx = 0, Ny = 0, Nz = 0, total = 0;
     aeros_lbm_config user_cfg{};   // the parameters init was called with (tau and U0 unsanitised)
     float minX = 0, minY = 0, minZ = 0, csX = 0.1f, csY = 0.1f, csZ = 0.1f;
@@ -337,17 +338,22 @@ public:
     int yLo = 1, yHi = 0; float H = 1, tauT = 0.7f;
     float tNu = 0, tAlpha = 0, tRa = 0, tPr = 0, tG = 0, tNusselt = 0, tMeanT = 0.5f;
 
+    bool quantised() const {
+        return storage == AEROS_FP16 || storage == AEROS_BF16 || storage == AEROS_FP32_FP64C;
+    }
     R qz(R x, int q) const {   // storage quantisation of a population
+        if (storage == AEROS_FP32_FP64C) return K<R>::w(q) + (R)(float)(x - K<R>::w(q));
         if (storage == AEROS_FP16) return K<R>::w(q) + (R)q_half((float)(x - K<R>::w(q)));
         if (storage == AEROS_BF16) return K<R>::w(q) + (R)q_bf16((float)(x - K<R>::w(q)));
         return x;
     }
     R qu(R x) const {          // storage quantisation of the velocity stencil input
+        if (storage == AEROS_FP32_FP64C) return (R)(float)x;
         if (storage == AEROS_FP16 || storage == AEROS_BF16) return (R)q_half((float)x);
         return x;
     }
     void quantize_cell(std::vector<R>& a, int cell) {
-        if (storage != AEROS_FP16 && storage != AEROS_BF16) return;
+        if (!quantised()) return;
         for (int q = 0; q < Q; ++q) a[(size_t)cell * Q + q] = qz(a[(size_t)cell * Q + q], q);
     }
 
@@ -543,7 +549,7 @@ public:
             for (int q = 0; q < Q; ++q) f[(size_t)cc * Q + q] = eq_guarded<R>(q, R(1), u0, u1, u2, us);
         }
         std::fill(fNext.begin(), fNext.end(), R(0));
-        if (storage == AEROS_FP16 || storage == AEROS_BF16)
+        if (quantised())
             for (int cc = 0; cc < total; ++cc) { quantize_cell(f, cc); quantize_cell(fNext, cc); }
 
         // tau from Re or from the user (aeros/src/lbm.cpp:741-815)
@@ -813,7 +819,7 @@ public:
             if (cfg.use_les) {
                 std::vector<R> qux, quy, quz;
                 const std::vector<R>* pu[3] = {&ux, &uy, &uz};
-                if (storage == AEROS_FP16 || storage == AEROS_BF16) {
+                if (quantised()) {
                     qux.resize(total); quy.resize(total); quz.resize(total);
                     for (int cc = 0; cc < total; ++cc) { qux[cc] = qu(ux[cc]); quy[cc] = qu(uy[cc]); quz[cc] = qu(uz[cc]); }
                     pu[0] = &qux; pu[1] = &quy; pu[2] = &quz;
@@ -1265,7 +1271,11 @@ void* ora_create(const aeros_lbm_config* c, int* err) {
     if (!h) { if (err) *err = AEROS_ERR_NOMEM; return nullptr; }
     h->prec = c->precision;
     int rc;
-    if (c->precision == AEROS_FP64) { h->s64.reset(new Sim<double>()); rc = h->s64->init(*c); }
+    if (c->precision == AEROS_FP64 || c->precision == AEROS_FP32_FP64C) {
+        h->s64.reset(new Sim<double>());
+        h->s64->storage = c->precision;
+        rc = h->s64->init(*c);
+    }
     else { h->s32.reset(new Sim<float>()); h->s32->storage = c->precision; rc = h->s32->init(*c); }
     if (err) *err = rc;
     if (rc != AEROS_OK) { delete h; return nullptr; }

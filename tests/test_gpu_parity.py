@@ -110,9 +110,15 @@ SIXTEEN_BIT_CASES = {
 W19 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, dtype=np.float32).astype(np.float64)
 
 
+# the FP64-collision mode rounds deviations from the double weights
+W19_64 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, dtype=np.float64)
+
+
 def ulp16(d, prec):
-    """Spacing of the 16-bit format at the stored deviation d (fp16: 10 mantissa
-    bits, subnormal below 2^-14; bf16: 7 mantissa bits)."""
+    """Spacing of the storage format at the stored deviation d (fp16: 10 mantissa
+    bits, subnormal below 2^-14; bf16: 7 mantissa bits; float for fp32_fp64c)."""
+    if prec == "fp32_fp64c":
+        return np.spacing(np.abs(d).astype(np.float32)).astype(np.float64)
     a = np.abs(d)
     e = np.floor(np.log2(np.where(a > 0, a, 1.0)))
     if prec == "fp16":
@@ -120,7 +126,7 @@ def ulp16(d, prec):
     return 2.0 ** (e - (10 if prec == "fp16" else 7))
 
 
-@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("prec", ["fp16", "bf16", "fp32_fp64c"])
 @pytest.mark.parametrize("name", sorted(SIXTEEN_BIT_CASES))
 def test_16bit_stored_populations_match_the_quantised_oracle(oracle, prec, name):
     """The 16-bit collision is the reference's relaxation folded into fused
@@ -130,16 +136,18 @@ def test_16bit_stored_populations_match_the_quantised_oracle(oracle, prec, name)
     tie.  After the first step every stored population is the oracle's; after the
     second, at least 99.5 % are and the others lie one unit in the last place
     away (the second step is the first with the Smagorinsky stencil and the
-    relaxation of a non-equilibrium state)."""
+    relaxation of a non-equilibrium state).  The FP32-storage mode with FP64
+    collision (fp32_fp64c) is held to the same rule in float deviations."""
     build, _, _, _ = SIXTEEN_BIT_CASES[name]
+    wq = W19_64 if prec == "fp32_fp64c" else W19
     case = dict(build(), precision=prec)
     G, O = make_pair(oracle, **case)
     fluid = (G.field("solid") < 0.5).reshape(-1)
     for k in (1, 2):
         G.step(1)
         O.step(1)
-        dg = G.populations().reshape(-1, 19)[fluid] - W19
-        dq = O.populations().reshape(-1, 19)[fluid] - W19
+        dg = G.populations().reshape(-1, 19)[fluid] - wq
+        dq = O.populations().reshape(-1, 19)[fluid] - wq
         diff = np.abs(dg - dq)
         same = float((diff == 0).mean())
         print(f"{name} {prec} step {k}: {same:.4%} of stored populations identical", flush=True)
@@ -150,7 +158,7 @@ def test_16bit_stored_populations_match_the_quantised_oracle(oracle, prec, name)
             assert np.all(diff <= ulp16(dq, prec)), float((diff / ulp16(dq, prec)).max())
 
 
-@pytest.mark.parametrize("prec", ["fp16", "bf16"])
+@pytest.mark.parametrize("prec", ["fp16", "bf16", "fp32_fp64c"])
 @pytest.mark.parametrize("name", sorted(SIXTEEN_BIT_CASES))
 def test_16bit_storage_against_quantised_and_fp64_oracle(oracle, prec, name):
     """Over a long run the GPU's 16-bit storage error against the FP64 oracle stays
