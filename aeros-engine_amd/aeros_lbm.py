@@ -22,10 +22,11 @@ import numpy as np
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "build" / "libaeros_lbm.so"
 
-FP32, FP64, FP16, BF16, FP32_FP64C = 0, 1, 2, 3, 4
+FP32, FP64, FP16, BF16, FP32_FP64C, FP16_EQ, BF16_EQ = 0, 1, 2, 3, 4, 5, 6
 # Domain faces (Config.free_slip): x = 0, x = Nx-1, y = 0, y = Ny-1, z = 0, z = Nz-1.
 FACE_X0, FACE_X1, FACE_Y0, FACE_Y1, FACE_Z0, FACE_Z1 = 1, 2, 4, 8, 16, 32
-PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp16": FP16, "bf16": BF16, "fp32_fp64c": FP32_FP64C}
+PRECISIONS = {"fp32": FP32, "fp64": FP64, "fp16": FP16, "bf16": BF16, "fp32_fp64c": FP32_FP64C,
+              "fp16_eq": FP16_EQ, "bf16_eq": BF16_EQ}
 
 (FIELD_RHO, FIELD_UX, FIELD_UY, FIELD_UZ, FIELD_UX_WORLD, FIELD_UY_WORLD, FIELD_UZ_WORLD,
  FIELD_PRESSURE, FIELD_TKE, FIELD_STRAIN, FIELD_SOLID, FIELD_GROUND, FIELD_WALLDIST,
@@ -214,14 +215,16 @@ class ServeConfig(C.Structure):
     """aeros_serve_config (include/aeros_serve.h)."""
     _fields_ = [("struct_size", C.c_int32), ("precision", C.c_int32), ("memory_budget", C.c_int64),
                 ("max_cases", C.c_int32), ("round_steps", C.c_int32), ("cd_tolerance", C.c_double),
-                ("max_steps", C.c_int32), ("cv_half", C.c_float), ("device", C.c_int32)]
+                ("max_steps", C.c_int32), ("cv_half", C.c_float), ("device", C.c_int32),
+                ("tick_steps", C.c_int32), ("cd_average", C.c_int32)]
 
 
 class ServeResult(C.Structure):
     _fields_ = [("state", C.c_int32), ("steps", C.c_int32), ("rounds", C.c_int32), ("resets", C.c_int32),
                 ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("error", C.c_int32),
                 ("cd", C.c_double), ("cd_prev", C.c_double), ("area", C.c_float), ("nusselt", C.c_float),
-                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double), ("bytes", C.c_int64)]
+                ("t_submit", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double), ("bytes", C.c_int64),
+                ("tick_admit", C.c_int64), ("tick_done", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -229,7 +232,7 @@ class ServeResult(C.Structure):
 
 class ServeStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64), ("rounds", C.c_int64),
-                ("launches", C.c_int64), ("resident", C.c_int32), ("queued", C.c_int32),
+                ("launches", C.c_int64), ("ticks", C.c_int64), ("resident", C.c_int32), ("queued", C.c_int32),
                 ("peak_resident", C.c_int32), ("bytes_in_use", C.c_int64), ("peak_bytes", C.c_int64),
                 ("busy_seconds", C.c_double), ("cell_steps", C.c_int64)]
 

@@ -223,6 +223,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
                 if (!std::isfinite(fe)) fe = w_of<R>(q) * R(1);
                 P.feq_in[q] = fe;
             }
+            for (int q = 0; q < Q; ++q) P.st_shift[q] = kEqShift<S> ? P.feq_in[q] : w_of<R>(q);
         }
         float cc = U0 * 0.8f;   // aeros/src/lbm.cpp:1467-1469
         if (cc > 0.5f) cc = 0.5f;
@@ -330,7 +331,7 @@ template <class R, class S, class U> struct Engine final : EngineBase {
             du3 = u3.as<float>();
         }
         hipLaunchKernelGGL((k_init<R, S>), dim3(grid_for(n)), dim3(256), 0, st, f[0].as<S>(), f[1].as<S>(),
-                           mask.as<uint32_t>(), du3, n, (R)s.inlet[0], (R)s.inlet[1], (R)s.inlet[2]);
+                           mask.as<uint32_t>(), du3, n, (R)s.inlet[0], (R)s.inlet[1], (R)s.inlet[2], params());
         HIP_TRY(hipGetLastError());
         // fields: rho 1, u 0, tke 0, strain 0 (aeros/src/lbm.cpp:567-580)
         if ((rc = fill<R>(fld[0], n, R(1)))) return rc;
@@ -477,6 +478,11 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         Status S_{};
         HIP_TRY(hipMemcpyAsync(&S_, status.p, sizeof S_, hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
+        return finish_status(S_, start, nsteps, was_pre);
+    }
+    // The same with the status already on the host (served mode reads every
+    // case's status with one copy).
+    int finish_status(Status S_, int start, int nsteps, bool was_pre) {
         if (S_.abort == 0) {
             cur = start + nsteps;
             pre = false;
@@ -666,7 +672,8 @@ template <class R, class S, class U> struct Engine final : EngineBase {
         if (rc) return rc;
         const int p = cur & 1;
         if (pre) {
-            hipLaunchKernelGGL((k_widen<R, S>), dim3(grid_for(n)), dim3(256), 0, st, f[p].as<S>(), tmp.as<R>(), n);
+            hipLaunchKernelGGL((k_widen<R, S>), dim3(grid_for(n)), dim3(256), 0, st, f[p].as<S>(), tmp.as<R>(), n,
+                               params());
         } else {
             StepParams<R> P = params();
             hipLaunchKernelGGL((k_pull_only<R, S>), dim3(grid_for(n)), dim3(256), 0, st, f[p ^ 1].as<S>(), f[p].as<S>(),
@@ -798,17 +805,28 @@ template <class R, class S, class U> struct Engine final : EngineBase {
     }
 
     // lbmControlVolumeForce box (aeros/src/lbm.cpp:2188-2198)
-    int cv_force(float hxf, float hyf, float hzf, double out[3]) override {
+    // The control volume of lbmControlVolumeForce (aeros/src/lbm.cpp:2183-2200):
+    // half extents as fractions of the domain around its centre, kept one
+    // cell inside the shell; with the fields it reads.
+    CvCase<R> cv_case(float hxf, float hyf, float hzf) const {
         auto side = [](int v, int N) { return v < 1 ? 1 : (v > N - 2 ? N - 2 : v); };
         const int nx = s.nx, ny = s.ny, nz = s.nz;
         const int cx = (int)(nx * 0.5f), cy = (int)(ny * 0.5f), cz = (int)(nz * 0.5f);
         const int hx = std::max(2, (int)(nx * hxf)), hy = std::max(2, (int)(ny * hyf)), hz = std::max(2, (int)(nz * hzf));
-        const int x0 = side(cx - hx, nx), x1 = side(cx + hx, nx), y0 = side(cy - hy, ny), y1 = side(cy + hy, ny);
-        const int z0 = side(cz - hz, nz), z1 = side(cz + hz, nz);
-        const int gb = 256;
-        hipLaunchKernelGGL((k_cv_force<R>), dim3(gb), dim3(BLOCK), 0, st, mask.as<uint32_t>(), solid_u8.as<uint8_t>(),
-                           fld[0].as<R>(), fld[1].as<R>(), fld[2].as<R>(), fld[3].as<R>(), nx, ny, nz, x0, x1, y0, y1,
-                           z0, z1, scratch.as<double>());
+        CvCase<R> c;
+        c.solid = solid_u8.as<uint8_t>();
+        c.rho = fld[0].as<R>(); c.ux = fld[1].as<R>(); c.uy = fld[2].as<R>(); c.uz = fld[3].as<R>();
+        c.nx = nx; c.ny = ny; c.nz = nz;
+        c.x0 = side(cx - hx, nx); c.x1 = side(cx + hx, nx); c.y0 = side(cy - hy, ny); c.y1 = side(cy + hy, ny);
+        c.z0 = side(cz - hz, nz); c.z1 = side(cz + hz, nz);
+        return c;
+    }
+
+    int cv_force(float hxf, float hyf, float hzf, double out[3]) override {
+        const CvCase<R> c = cv_case(hxf, hyf, hzf);
+        const int gb = kCvBlocks;
+        hipLaunchKernelGGL((k_cv_force<R>), dim3(gb), dim3(BLOCK), 0, st, c.solid, c.rho, c.ux, c.uy, c.uz, c.nx, c.ny,
+                           c.nz, c.x0, c.x1, c.y0, c.y1, c.z0, c.z1, scratch.as<double>());
         HIP_TRY(hipGetLastError());
         std::vector<double> hp((size_t)gb * 3);
         HIP_TRY(hipMemcpyAsync(hp.data(), scratch.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -877,6 +895,8 @@ EngineBase* make_engine(aeros_lbm* h) {
         case AEROS_FP16: return new (std::nothrow) Engine<float, _Float16, _Float16>(h);
         case AEROS_BF16: return new (std::nothrow) Engine<float, bf16_t, _Float16>(h);
         case AEROS_FP32_FP64C: return new (std::nothrow) Engine<double, float, float>(h);
+        case AEROS_FP16_EQ: return new (std::nothrow) Engine<float, h16e_t, _Float16>(h);
+        case AEROS_BF16_EQ: return new (std::nothrow) Engine<float, b16e_t, _Float16>(h);
         default: return nullptr;
     }
 }
@@ -984,7 +1004,7 @@ aeros_lbm* aeros_lbm_create(const aeros_lbm_config* cfg, int* err) {
     if (!cfg) return fail(set_err(AEROS_ERR_ARG, "null config"));
     if (cfg->struct_size != (int32_t)sizeof(aeros_lbm_config))
         return fail(set_err(AEROS_ERR_ARG, "config struct_size %d != %d", cfg->struct_size, (int)sizeof(aeros_lbm_config)));
-    if (cfg->precision < AEROS_FP32 || cfg->precision > AEROS_FP32_FP64C)
+    if (cfg->precision < AEROS_FP32 || cfg->precision > AEROS_BF16_EQ)
         return fail(set_err(AEROS_ERR_ARG, "unknown precision %d", cfg->precision));
     {
         char msg[160];

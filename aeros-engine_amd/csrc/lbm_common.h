@@ -144,6 +144,9 @@ template <class R> struct StepParams {
     // output of populations in [-1, 10] far below FLT_MAX, so that the
     // non-finite fix-up can be skipped for such cells
     int bounded;
+    // the shift of the stored populations of the equilibrium-shifted 16-bit
+    // storage: the inflow equilibrium feq_in (w_q for the other storages)
+    R st_shift[Q];
 };
 
 }  // namespace aeros

@@ -22,6 +22,8 @@
 
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "lbm_common.h"
 
 namespace aeros {
@@ -73,7 +75,12 @@ template <class T> __device__ inline gref<T> at(T* base, uint32_t i) {
 template <class T> __device__ inline void st_nt(AEROS_GLOBAL T& dst, T v) { __builtin_nontemporal_store(v, &dst); }
 
 template <class S, class R> struct Pop;
+// Each storage: M, the element as memory holds it; dec / enc between it and
+// the arithmetic type; ld / st of element i of an array.
 template <> struct Pop<float, float> {
+    using M = float;
+    __device__ static float dec(float s, float) { return s; }
+    __device__ static float enc(float v, float) { return v; }
     __device__ static float ld(const float* p, uint32_t i, float) { return at(p, i); }
     __device__ static void st(float* p, uint32_t i, float v, float) { st_nt(at(p, i), v); }
 };
@@ -81,6 +88,9 @@ template <> struct Pop<float, float> {
 // follows the deviation, not w, so the small per-step changes of a slow
 // decay are kept (a float f near w = 1/18 has a spacing of 3.7e-9)
 template <> struct Pop<float, double> {
+    using M = float;
+    __device__ static double dec(float s, double w) { return w + (double)s; }
+    __device__ static float enc(double v, double w) { return (float)(v - w); }
     __device__ static double ld(const float* p, uint32_t i, double w) { return w + (double)at(p, i); }
     __device__ static void st(float* p, uint32_t i, double v, double w) { st_nt(at(p, i), (float)(v - w)); }
 };
@@ -102,6 +112,9 @@ __device__ inline _Float16 half_minus(float v, float w) {
     return r;
 }
 template <> struct Pop<_Float16, float> {   // deviation from w_q, IEEE half, round to nearest even
+    using M = _Float16;
+    __device__ static float dec(_Float16 h, float w) { return half_plus(h, w); }
+    __device__ static _Float16 enc(float v, float w) { return half_minus(v, w); }
     __device__ static float ld(const _Float16* p, uint32_t i, float w) { return half_plus(at(p, i), w); }
     __device__ static void st(_Float16* p, uint32_t i, float v, float w) { st_nt(at(p, i), half_minus(v, w)); }
 };
@@ -110,9 +123,50 @@ __device__ inline uint16_t f_to_bf16(float x) {   // round to nearest even (one 
     return __builtin_bit_cast(uint16_t, (__bf16)x);
 }
 template <> struct Pop<bf16_t, float> {     // deviation from w_q, bfloat16
+    using M = uint16_t;
+    __device__ static float dec(uint16_t b, float w) { return w + bf16_to_f(b); }
+    __device__ static uint16_t enc(float v, float w) { return f_to_bf16(v - w); }
     __device__ static float ld(const bf16_t* p, uint32_t i, float w) { return w + bf16_to_f(at(p, i).bits); }
     __device__ static void st(bf16_t* p, uint32_t i, float v, float w) { st_nt(at(p, i).bits, f_to_bf16(v - w)); }
 };
+// Equilibrium-shifted 16-bit storage: the stored value is f - e_q, where e_q
+// is the inflow equilibrium (passed where the storages above take w_q), so
+// the free stream stores values near zero.  The half form is scaled by 2^12
+// (deviations down to 1.5e-8 stay normal; |f - e| <= 11 keeps it below the
+// half range): stored = half(float(f - e) * 4096), read = h * 2^-12 + e, each
+// rounded once as written.
+struct h16e_t { _Float16 h; };
+struct b16e_t { uint16_t bits; };
+__device__ inline float half_scaled_plus(_Float16 h, float e) {
+    float r;
+    asm("v_fma_mix_f32 %0, %1, %2, %3 op_sel_hi:[1,0,0]" : "=v"(r) : "v"(h), "v"(0x1p-12f), "v"(e));
+    return r;
+}
+__device__ inline _Float16 half_scaled(float d) {
+    _Float16 r;
+    asm("v_fma_mixlo_f16 %0, %1, %2, 0" : "=v"(r) : "v"(d), "v"(4096.0f));
+    return r;
+}
+template <> struct Pop<h16e_t, float> {
+    using M = _Float16;
+    __device__ static float dec(_Float16 h, float e) { return half_scaled_plus(h, e); }
+    __device__ static _Float16 enc(float v, float e) { return half_scaled(v - e); }
+    __device__ static float ld(const h16e_t* p, uint32_t i, float e) { return half_scaled_plus(at(p, i).h, e); }
+    __device__ static void st(h16e_t* p, uint32_t i, float v, float e) { st_nt(at(p, i).h, half_scaled(v - e)); }
+};
+template <> struct Pop<b16e_t, float> {
+    using M = uint16_t;
+    __device__ static float dec(uint16_t b, float e) { return e + bf16_to_f(b); }
+    __device__ static uint16_t enc(float v, float e) { return f_to_bf16(v - e); }
+    __device__ static float ld(const b16e_t* p, uint32_t i, float e) { return e + bf16_to_f(at(p, i).bits); }
+    __device__ static void st(b16e_t* p, uint32_t i, float v, float e) { st_nt(at(p, i).bits, f_to_bf16(v - e)); }
+};
+template <class S> constexpr bool kEqShift = std::is_same<S, h16e_t>::value || std::is_same<S, b16e_t>::value;
+// The value a stored population of direction q is taken relative to.
+template <class S, class R> __device__ __forceinline__ R sshift(const StepParams<R>& P, int q) {
+    if constexpr (kEqShift<S>) return P.st_shift[q];
+    else return w_of<R>(q);
+}
 
 // A lattice seen by the step kernel: direction planes of n cells, element
 // plane + idx with plane = q n (less the pull shift) >= 0.  Element types up
@@ -566,18 +620,18 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
         R f[Q];
         if (F.first) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) f[q] = lin.ld(cell, (uint32_t)q * n, w_of<R>(q));
+            for (int q = 0; q < Q; ++q) f[q] = lin.ld(cell, (uint32_t)q * n, sshift<S>(P, q));
         } else if (!(m & M_FACES)) {
             // interior: the source of q is cell - c_q; a uniform shifted base per direction
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const int off = cx_of(q) + cy_of(q) * nx + cz_of(q) * (int)sz;
-                f[q] = lin.ld(cell, (uint32_t)q * n - (uint32_t)off, w_of<R>(q));
+                f[q] = lin.ld(cell, (uint32_t)q * n - (uint32_t)off, sshift<S>(P, q));
             }
             if (m & M_LINKS) {   // half-way bounce-back: the cell's own opposite population
 #pragma unroll
                 for (int q = 1; q < Q; ++q)
-                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, w_of<R>(opp_of(q)));
+                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, sshift<S>(P, opp_of(q)));
             }
         } else if (m & M_PULL_INLET) {
 #pragma unroll
@@ -588,7 +642,7 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
             const uint32_t ci = (uint32_t)((int64_t)cell - P.out_d1), ci2 = (uint32_t)((int64_t)cell - P.out_d2);
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                const R w = w_of<R>(q);
+                const R w = sshift<S>(P, q);
                 const R fo = lin.ld(cell, (uint32_t)q * n, w);
                 const R fi = lin.ld(ci, (uint32_t)q * n, w);
                 const R fi2 = lin.ld(ci2, (uint32_t)q * n, w);
@@ -602,12 +656,12 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
             for (int q = 0; q < Q; ++q) {
                 const int off = cx_of(q) + cy_of(q) * nx + cz_of(q) * (int)sz;
                 const uint32_t src = (m & out_faces(q)) ? cell : cell - (uint32_t)off;
-                f[q] = lin.ld(src, (uint32_t)q * n, w_of<R>(q));
+                f[q] = lin.ld(src, (uint32_t)q * n, sshift<S>(P, q));
             }
             if (m & M_LINKS) {
 #pragma unroll
                 for (int q = 1; q < Q; ++q)
-                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, w_of<R>(opp_of(q)));
+                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, sshift<S>(P, opp_of(q)));
             }
         } else {
             // domain shell on a periodic or free-slip face: a source beyond a
@@ -634,7 +688,7 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
                     const int dq = ((int)rx + (int)ry + (int)rz >= 2)
                                        ? opp_of(q)
                                        : (rx ? mirror_of(q, 1) : (ry ? mirror_of(q, 2) : (rz ? mirror_of(q, 4) : q)));
-                    f[q] = lin.ld(out ? cell : src, (uint32_t)(out ? q : dq) * n, w_of<R>(q));
+                    f[q] = lin.ld(out ? cell : src, (uint32_t)(out ? q : dq) * n, sshift<S>(P, out ? q : dq));
                 }
             } else {
 #pragma unroll
@@ -645,13 +699,13 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
                     const uint32_t wrap = ((m & xf) ? (uint32_t)(cx_of(q) * nx) : 0u) +
                                           ((m & zf) ? (uint32_t)(cz_of(q) * nz) * sz : 0u);
                     const uint32_t src = out ? cell : cell - (uint32_t)off + wrap;
-                    f[q] = lin.ld(src, (uint32_t)q * n, w_of<R>(q));
+                    f[q] = lin.ld(src, (uint32_t)q * n, sshift<S>(P, q));
                 }
             }
             if (m & M_LINKS) {   // the link bits refer to the wrapped or reflected source
 #pragma unroll
                 for (int q = 1; q < Q; ++q)
-                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, w_of<R>(opp_of(q)));
+                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, sshift<S>(P, opp_of(q)));
             }
         }
 
@@ -865,7 +919,7 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
         }
 
 #pragma unroll
-        for (int q = 0; q < Q; ++q) lout.st(cell, (uint32_t)q * n, f[q], w_of<R>(q));
+        for (int q = 0; q < Q; ++q) lout.st(cell, (uint32_t)q * n, f[q], sshift<S>(P, q));
 
         // ---------------- velocity and diagnostics (aeros/src/lbm.cpp:1266-1305)
         R vx = ux, vy = uy, vz = uz;   // the stored velocity includes F/(2 rho)
@@ -1027,7 +1081,7 @@ struct BlockMap {
     uint32_t ci, blk;
 };
 
-template <class R, class S, class U, bool LES, bool THERM, bool FORCE>
+template <class R, class S, class U, bool LES, bool THERM, bool FORCE, bool SLIP>
 __global__ __launch_bounds__(BLOCK) void k_step_batch(const BatchCase<R>* __restrict__ cases,
                                                       const BlockMap* __restrict__ map, int i, int nsteps) {
     __shared__ RedLds L;
@@ -1039,22 +1093,26 @@ __global__ __launch_bounds__(BLOCK) void k_step_batch(const BatchCase<R>* __rest
     const StepFlags F{(C.pre && i == 0) ? 1 : 0, (i == nsteps - 1) ? 1 : 0, 0, k, i > 0 ? 1 : 0};
     StepBuffers B = C.buf[k & 1];
     if (F.first) B.f_in = B.f_out;
-    step_grid<R, S, U, LES, THERM, FORCE>(C.P, B, F, bm.blk, C.nblocks, L);
+    step_grid<R, S, U, LES, THERM, FORCE, SLIP>(C.P, B, F, bm.blk, C.nblocks, L);
 }
-// The end of a served round: one block per case folds its last step.
+// The end of a served tick: one block per case folds its last step and
+// copies the case's status to out[case], so that one copy brings every
+// case's status to the host.
 template <class R>
-__global__ __launch_bounds__(BLOCK) void k_fold_batch(const BatchCase<R>* cases, int nsteps) {
+__global__ __launch_bounds__(BLOCK) void k_fold_batch(const BatchCase<R>* cases, int nsteps, Status* out) {
     __shared__ RedLds L;
     const BatchCase<R>& C = cases[blockIdx.x];
     const int k = C.start + nsteps - 1;
     fold_last<R>(C.P, C.buf[k & 1], k, C.nblocks, L);
+    if (threadIdx.x == 0) out[blockIdx.x] = *C.buf[k & 1].status;   // thread 0 wrote the fold's result
 }
 
 // ------------------------------------------------------------ other kernels
 // Fill both lattices: buffer A gets the initial equilibria, buffer B zeros
 // (the reference's f / fNext after initLBM, aeros/src/lbm.cpp:581-582, 690-739).
 template <class R, class S>
-__global__ void k_init(S* fa, S* fb, const uint32_t* mask, const float* u3, int64_t n, R u0, R u1, R u2) {
+__global__ void k_init(S* fa, S* fb, const uint32_t* mask, const float* u3, int64_t n, R u0, R u1, R u2,
+                       StepParams<R> P) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
         const bool sol = mask[c] & M_SOLID;
         R a = R(0), b = R(0), d = R(0);
@@ -1068,8 +1126,8 @@ __global__ void k_init(S* fa, S* fb, const uint32_t* mask, const float* u3, int6
             a *= s; b *= s; d *= s; us = a * a + b * b + d * d;
         }
         for (int q = 0; q < Q; ++q) {
-            Pop<S, R>::st(fa + (size_t)q * n, (uint32_t)c, feq_guarded<R>(q, R(1), a, b, d, us), w_of<R>(q));
-            Pop<S, R>::st(fb + (size_t)q * n, (uint32_t)c, R(0), w_of<R>(q));
+            Pop<S, R>::st(fa + (size_t)q * n, (uint32_t)c, feq_guarded<R>(q, R(1), a, b, d, us), sshift<S>(P, q));
+            Pop<S, R>::st(fb + (size_t)q * n, (uint32_t)c, R(0), sshift<S>(P, q));
         }
     }
 }
@@ -1081,7 +1139,7 @@ __global__ void k_reset(S* f, const uint32_t* mask, int64_t n, StepParams<R> P, 
     constexpr int US = VelAos<U>::stride;
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x) {
         if (mask[c] & M_SOLID) continue;
-        for (int q = 0; q < Q; ++q) Pop<S, R>::st(f + (size_t)q * n, (uint32_t)c, P.feq_in[q], w_of<R>(q));
+        for (int q = 0; q < Q; ++q) Pop<S, R>::st(f + (size_t)q * n, (uint32_t)c, P.feq_in[q], sshift<S>(P, q));
         if (uaos) {
             U* u = uaos + (size_t)c * US;
             u[0] = (U)P.in_u[0]; u[1] = (U)P.in_u[1]; u[2] = (U)P.in_u[2];
@@ -1144,7 +1202,7 @@ __global__ void k_pull_only(const S* fin_, const S* fsolid, R* out, const uint32
     for (int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; cell < n; cell += (int64_t)gridDim.x * blockDim.x) {
         const uint32_t m = mask[cell];
         if (m & M_SOLID) {
-            for (int q = 0; q < Q; ++q) out[q * n + cell] = Pop<S, R>::ld(fsolid + (size_t)q * n, (uint32_t)cell, w_of<R>(q));
+            for (int q = 0; q < Q; ++q) out[q * n + cell] = Pop<S, R>::ld(fsolid + (size_t)q * n, (uint32_t)cell, sshift<S>(P, q));
             continue;
         }
         if (m & M_PULL_INLET) {
@@ -1154,7 +1212,7 @@ __global__ void k_pull_only(const S* fin_, const S* fsolid, R* out, const uint32
         if (m & M_PULL_OUTLET) {
             const int64_t ci = cell - P.out_d1, ci2 = cell - P.out_d2;
             for (int q = 0; q < Q; ++q) {
-                const R w = w_of<R>(q);
+                const R w = sshift<S>(P, q);
                 const S* b = fin_ + (size_t)q * n;
                 const R fo = Pop<S, R>::ld(b, (uint32_t)cell, w), fi = Pop<S, R>::ld(b, (uint32_t)ci, w);
                 const R fi2 = Pop<S, R>::ld(b, (uint32_t)ci2, w);
@@ -1169,7 +1227,7 @@ __global__ void k_pull_only(const S* fin_, const S* fsolid, R* out, const uint32
             const bool bb = in && ((m >> q) & 1u);
             const int64_t src = (!in || bb) ? cell : ((int64_t)szz * ny + syy) * nx + sx;
             const int dq = bb ? opp_of(q) : (in ? mq : q);
-            out[q * n + cell] = Pop<S, R>::ld(fin_ + (size_t)dq * n, (uint32_t)src, w_of<R>(dq));
+            out[q * n + cell] = Pop<S, R>::ld(fin_ + (size_t)dq * n, (uint32_t)src, sshift<S>(P, dq));
         }
         (void)sy;
     }
@@ -1177,9 +1235,9 @@ __global__ void k_pull_only(const S* fin_, const S* fsolid, R* out, const uint32
 
 // Expand storage to R (populations of a pre-collision state).
 template <class R, class S>
-__global__ void k_widen(const S* in, R* out, int64_t n) {
+__global__ void k_widen(const S* in, R* out, int64_t n, StepParams<R> P) {
     for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < n; c += (int64_t)gridDim.x * blockDim.x)
-        for (int q = 0; q < Q; ++q) out[q * n + c] = Pop<S, R>::ld(in + (size_t)q * n, (uint32_t)c, w_of<R>(q));
+        for (int q = 0; q < Q; ++q) out[q * n + c] = Pop<S, R>::ld(in + (size_t)q * n, (uint32_t)c, sshift<S>(P, q));
 }
 
 // thermalUpload / thermalResetField populations: g = g_eq(T, 0) = w T.
@@ -1289,9 +1347,9 @@ __global__ void k_post_fold(const PostPart* parts, int nparts, int valid, float*
 // cell's float term is formed as the reference forms it and accumulated in
 // double; per-block partials are folded in block order by the host.
 template <class R>
-__global__ void k_cv_force(const uint32_t* solid_mask, const uint8_t* solid, const R* rho, const R* ux, const R* uy,
-                           const R* uz, int nx, int ny, int nz, int x0, int x1, int y0, int y1, int z0, int z1,
-                           double* partial) {
+__device__ __forceinline__ void cv_force_block(const uint8_t* solid, const R* rho, const R* ux, const R* uy,
+                                               const R* uz, int nx, int ny, int nz, int x0, int x1, int y0, int y1,
+                                               int z0, int z1, double* partial) {
     __shared__ double s[3][BLOCK];
     const int tid = threadIdx.x;
     const int64_t nyz = (int64_t)(z1 - z0 + 1) * (y1 - y0 + 1), nzx = (int64_t)(z1 - z0 + 1) * (x1 - x0 + 1),
@@ -1325,10 +1383,31 @@ __global__ void k_cv_force(const uint32_t* solid_mask, const uint8_t* solid, con
         fy += (double)(pp * nyv + rr * vy * un);
         fz += (double)(pp * nzv + rr * vz * un);
     }
-    (void)solid_mask;
     s[0][tid] = fx; s[1][tid] = fy; s[2][tid] = fz;
     for (int k = 0; k < 3; ++k) lds_tree_sum(s[k], tid);
     if (tid == 0) { partial[blockIdx.x * 3] = s[0][0]; partial[blockIdx.x * 3 + 1] = s[1][0]; partial[blockIdx.x * 3 + 2] = s[2][0]; }
+}
+template <class R>
+__global__ void k_cv_force(const uint8_t* solid, const R* rho, const R* ux, const R* uy, const R* uz, int nx, int ny,
+                           int nz, int x0, int x1, int y0, int y1, int z0, int z1, double* partial) {
+    cv_force_block<R>(solid, rho, ux, uy, uz, nx, ny, nz, x0, x1, y0, y1, z0, z1, partial);
+}
+
+// The control-volume force of several lattices in one launch: blockIdx.y
+// selects the lattice, and each lattice's 256 blocks partition its faces
+// exactly as k_cv_force's grid of 256 does, so the partials (and their sum
+// in block order on the host) are the same bits as a k_cv_force call.
+constexpr int kCvBlocks = 256;   // blocks of a control-volume force launch (the partition of its faces)
+template <class R> struct CvCase {
+    const uint8_t* solid;
+    const R *rho, *ux, *uy, *uz;
+    int nx, ny, nz, x0, x1, y0, y1, z0, z1;
+};
+template <class R>
+__global__ void k_cv_force_batch(const CvCase<R>* cases, double* partial) {
+    const CvCase<R>& c = cases[blockIdx.y];
+    cv_force_block<R>(c.solid, c.rho, c.ux, c.uy, c.uz, c.nx, c.ny, c.nz, c.x0, c.x1, c.y0, c.y1, c.z0, c.z1,
+                      partial + (size_t)blockIdx.y * gridDim.x * 3);
 }
 
 // Exact checksum of the rho field: in the float modes every rho lies in
@@ -1698,8 +1777,13 @@ static hipError_t launch_step_t(const StepParams<R>& P, const StepBuffers& B, in
 template <class R, class S, class U>
 hipError_t launch_step(const StepParams<R>& P, const StepBuffers& B, int grid, hipStream_t s) {
     const bool force = P.fx != R(0) || P.fy != R(0) || P.fz != R(0);
-    if (P.slip_faces) {   // the set-up rejects free-slip walls together with the thermal lattice
-        if (P.thermal) return hipErrorInvalidValue;
+    if (P.slip_faces) {
+        if (P.thermal) {
+            if (P.les) return force ? launch_step_t<R, S, U, true, true, true, true>(P, B, grid, s)
+                                    : launch_step_t<R, S, U, true, true, false, true>(P, B, grid, s);
+            return force ? launch_step_t<R, S, U, false, true, true, true>(P, B, grid, s)
+                         : launch_step_t<R, S, U, false, true, false, true>(P, B, grid, s);
+        }
         if (P.les) return force ? launch_step_t<R, S, U, true, false, true, true>(P, B, grid, s)
                                 : launch_step_t<R, S, U, true, false, false, true>(P, B, grid, s);
         return force ? launch_step_t<R, S, U, false, false, true, true>(P, B, grid, s)
@@ -1717,22 +1801,28 @@ hipError_t launch_step(const StepParams<R>& P, const StepBuffers& B, int grid, h
                  : launch_step_t<R, S, U, false, false, false>(P, B, grid, s);
 }
 
-// Served mode: one launch over the cases of one (LES, thermal) group; served
-// cases carry no body force.
-template <class R, class S, class U, bool LES, bool THERM>
+// Served mode: one launch over the cases of one (LES, thermal, free-slip)
+// group; served cases carry no body force.
+template <class R, class S, class U, bool LES, bool THERM, bool SLIP>
 static hipError_t launch_batch_t(const BatchCase<R>* cases, const BlockMap* map, int nblocks, int i, int nsteps,
                                  hipStream_t s) {
-    hipLaunchKernelGGL((k_step_batch<R, S, U, LES, THERM, false>), dim3(nblocks), dim3(BLOCK), 0, s, cases, map, i,
-                       nsteps);
+    hipLaunchKernelGGL((k_step_batch<R, S, U, LES, THERM, false, SLIP>), dim3(nblocks), dim3(BLOCK), 0, s, cases, map,
+                       i, nsteps);
     return hipGetLastError();
 }
+template <class R, class S, class U, bool SLIP>
+static hipError_t launch_batch_s(bool les, bool thermal, const BatchCase<R>* cases, const BlockMap* map, int nblocks,
+                                 int i, int nsteps, hipStream_t s) {
+    if (les) return thermal ? launch_batch_t<R, S, U, true, true, SLIP>(cases, map, nblocks, i, nsteps, s)
+                            : launch_batch_t<R, S, U, true, false, SLIP>(cases, map, nblocks, i, nsteps, s);
+    return thermal ? launch_batch_t<R, S, U, false, true, SLIP>(cases, map, nblocks, i, nsteps, s)
+                   : launch_batch_t<R, S, U, false, false, SLIP>(cases, map, nblocks, i, nsteps, s);
+}
 template <class R, class S, class U>
-hipError_t launch_batch(bool les, bool thermal, const BatchCase<R>* cases, const BlockMap* map, int nblocks, int i,
-                        int nsteps, hipStream_t s) {
-    if (les) return thermal ? launch_batch_t<R, S, U, true, true>(cases, map, nblocks, i, nsteps, s)
-                            : launch_batch_t<R, S, U, true, false>(cases, map, nblocks, i, nsteps, s);
-    return thermal ? launch_batch_t<R, S, U, false, true>(cases, map, nblocks, i, nsteps, s)
-                   : launch_batch_t<R, S, U, false, false>(cases, map, nblocks, i, nsteps, s);
+hipError_t launch_batch(bool les, bool thermal, bool slip, const BatchCase<R>* cases, const BlockMap* map, int nblocks,
+                        int i, int nsteps, hipStream_t s) {
+    return slip ? launch_batch_s<R, S, U, true>(les, thermal, cases, map, nblocks, i, nsteps, s)
+                : launch_batch_s<R, S, U, false>(les, thermal, cases, map, nblocks, i, nsteps, s);
 }
 
 }  // namespace aeros

@@ -111,10 +111,8 @@ int check_boundaries(const aeros_lbm_config& c, char* msg, int msglen) {
         set_msg(msg, msglen, "free_slip names an unknown face or a face of a periodic axis");
         return AEROS_ERR_ARG;
     }
-    if (c.free_slip && c.thermal) {   // the D3Q7 lattice keeps its own shell rules (x and z wrap)
-        set_msg(msg, msglen, "free-slip walls are not available together with the thermal lattice");
-        return AEROS_ERR_ARG;
-    }
+    // with the thermal lattice, free-slip walls act on the D3Q19 lattice only:
+    // the D3Q7 lattice keeps its own shell rules (x and z wrap, y walls)
     return AEROS_OK;
 }
 

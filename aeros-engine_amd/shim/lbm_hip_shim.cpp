@@ -29,13 +29,17 @@ namespace {
 
 aeros_lbm* g_hip = nullptr;
 
-// Storage precision, AEROS_LBM_PRECISION = fp32 (default) | fp64 | fp16 | bf16.
+// Storage precision, AEROS_LBM_PRECISION = fp32 (default) | fp64 | fp16 | bf16 |
+// fp32_fp64c | fp16_eq | bf16_eq.
 int precision_from_env() {
     const char* p = std::getenv("AEROS_LBM_PRECISION");
     if (!p) return AEROS_FP32;
     if (!std::strcmp(p, "fp64")) return AEROS_FP64;
     if (!std::strcmp(p, "fp16")) return AEROS_FP16;
     if (!std::strcmp(p, "bf16")) return AEROS_BF16;
+    if (!std::strcmp(p, "fp32_fp64c")) return AEROS_FP32_FP64C;
+    if (!std::strcmp(p, "fp16_eq")) return AEROS_FP16_EQ;
+    if (!std::strcmp(p, "bf16_eq")) return AEROS_BF16_EQ;
     return AEROS_FP32;
 }
 

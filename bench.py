@@ -28,9 +28,13 @@ reference itself refuses lattices above 20 Mi cells) on the host cores for a
 bounded sample of steps of the same case, with the CPU model; the reference's own
 solver, built from its sources, is timed on the 256x128x104 case beside it.
 `secondary` holds the other precisions and cases (33.5M cells FP16/BF16, the
-256x128x104 case, the FP64 sphere, the coupled thermal lattice) and, unless
---no-studies, the precision and Nusselt studies; `served` the served-mode stream
-(solves/s, p50/p99 latency, against one-at-a-time solves).
+256x128x104 case, the FP64 sphere, the coupled thermal lattice), unless
+--no-studies the host-driver leg and the short precision studies, and with
+--studies the long ones (33.5M cells for 20k steps, the Nusselt number after
+300k steps); `served`, `served_fp16_eq` and `served_fp16` the served-mode stream
+in FP32, equilibrium-shifted FP16 and FP16 storage (saturation solves/s against
+one-at-a-time solves, p50/p99 latency at 0.7 of the saturation rate; the FP16
+line on the first 64 requests only).
 """
 from __future__ import annotations
 
@@ -117,7 +121,7 @@ def c1_results(A, W):
     return {"rho_sum_120": float("%.12g" % rho_sum), "cd_2000": round(cd, 4), "area_cells": area}
 
 
-def c3_nusselt(A, W, steps=300000, chunk=20000, precisions=("fp64", "fp32", "fp16")):
+def c3_nusselt(A, W, steps=300000, chunk=20000, precisions=("fp64", "fp32", "fp32_fp64c", "fp16")):
     """The C3 Rayleigh-Benard box run to steady state in each precision from the same
     seeded start: Nu at `steps` steps and its change over the last `chunk` steps.  The
     FP64 run stands for the FP64 oracle, which the FP64 kernel matches within 1e-12 and
@@ -147,7 +151,8 @@ def c3_nusselt(A, W, steps=300000, chunk=20000, precisions=("fp64", "fp32", "fp1
     return out
 
 
-def precision_study(A, W, scale=2.0, re=None, steps=6000, window=2000, every=100):
+def precision_study(A, W, scale=2.0, re=None, steps=20000, window=10000, every=100,
+                    precisions=("fp64", "fp32", "fp32_fp64c", "fp16", "bf16", "fp16_eq", "bf16_eq")):
     """One body, grid and Re in FP64, FP32, FP16 and BF16 storage from the same start:
     the drag coefficient (control volume 0.45/0.40/0.40 of the domain, A = projected
     solid cells along x) averaged over samples every `every` steps in the last `window`
@@ -157,12 +162,14 @@ def precision_study(A, W, scale=2.0, re=None, steps=6000, window=2000, every=100
     the 256x128x104 case does the same), so the study closes the y and z faces with
     free-slip walls.  scale 2, Re 1000 is the 33.5M-cell case; scale 1, Re 100 a steady
     companion whose drag settles, so its errors are those of the storage and not of an
-    unsteady wake."""
+    unsteady wake.  At scale 2 one flow-through is 512 / 0.08 = 6400 steps and the
+    wake sheds with a period of about 3000 steps, so 20k steps with the drag averaged
+    over the last 10k cover three flow-throughs and about three periods."""
     cfg0 = W.c2_config(A.voxelize_gpu, scale=scale, re=re)
     cfg0["free_slip"] = W.SIDE_WALLS
     fluid = (np.asarray(cfg0["voxels"]) == 0).reshape(-1)
     out, u64, cd64 = {}, None, None
-    for prec in ("fp64", "fp32", "fp16", "bf16"):
+    for prec in precisions:
         cfg = dict(cfg0, precision=prec)
         L = A.Lattice(A.default_config(**cfg))
         area = L.projected_area(0)
@@ -189,6 +196,33 @@ def precision_study(A, W, scale=2.0, re=None, steps=6000, window=2000, every=100
     out["note"] = (f"{dims[0]}x{dims[1]}x{dims[2]}, ellipsoid {30 * scale:g}/{12 * scale:g}/{12 * scale:g}, "
                    f"tau {cfg0['tau']:.5f}, TRT+LES, convective outlet, free-slip y/z faces; Cd mean of "
                    f"{window // every} samples over steps {steps - window + every}..{steps}; L2 after {steps} steps")
+    return out
+
+
+def c1_precision(A, W, steps=4000, window=1000, every=50):
+    """The C1 sphere (96x48x48, BGK, U0 0.05, free-slip side walls) in every storage:
+    Cd averaged over the last `window` steps against FP64, and the velocity L2."""
+    out, u64, cd64 = {}, None, None
+    for prec in ("fp64", "fp32", "fp32_fp64c", "fp16", "bf16", "fp16_eq", "bf16_eq"):
+        L = A.Lattice(A.default_config(**W.c1_sphere_fp64(precision=prec)))
+        area = L.projected_area(0)
+        L.step(steps - window)
+        cds = []
+        for _ in range(window // every):
+            L.step(every)
+            cds.append(L.cv_force(0.45, 0.40, 0.40)[0] / (0.5 * 0.05 * 0.05 * area))
+        u = np.stack([L.field(k).reshape(-1).astype(np.float64) for k in ("ux", "uy", "uz")])
+        L.close()
+        cd = float(np.mean(cds))
+        rec = {"cd_mean": round(cd, 5)}
+        if u64 is None:
+            u64, cd64 = u, cd
+        else:
+            rec["cd_rel_err_vs_fp64"] = float("%.4g" % ((cd - cd64) / cd64))
+            rec["u_rel_l2_vs_fp64"] = float("%.4g" % np.sqrt(((u - u64) ** 2).sum() / (u64 ** 2).sum()))
+        out[prec] = rec
+    out["note"] = f"96x48x48 sphere D 12, Cd mean of {window // every} samples over the last {window} of {steps} steps"
+    log(f"[precision c1] {out}")
     return out
 
 
@@ -297,24 +331,23 @@ def cpu_others(A, W, budget_s=6.0):
 
 
 # ------------------------------------------------------------------ served mode
-def solve_alone(A, W, cfg, round_steps, tol, max_steps):
-    """One request on its own lattice: rounds of round_steps, drag after each, the
-    served retire rule (|dCd| < tol over a round, or max_steps)."""
-    L = A.Lattice(A.default_config(**cfg, precision="fp16"))
+def solve_alone(A, W, cfg, precision, round_steps, tol, max_steps):
+    """One request on its own lattice with the plain solver: one call per round of
+    round_steps, the drag after each, retired at |dCd| < tol over a round or at
+    max_steps.  (The server steps the same launches in ticks; without a divergence
+    reset the populations, fields and drag are the same bits.)"""
+    L = A.Lattice(A.default_config(**cfg, precision=precision))
     area = L.projected_area(0)
     u0 = float(np.float32(np.sqrt(np.float32(cfg["U0"]) * np.float32(cfg["U0"]))))
     q = 0.5 * u0 * u0 * float(area)
-    steps, cd, prev, rounds = 0, 0.0, None, 0
+    cd, prev, rounds = 0.0, None, 0
     while True:
         L.step(round_steps)
         steps = L.scalars()["step"]   # the step counter, which a divergence reset moves back
         rounds += 1
-        cd = L.cv_force(W.C5_CV_HALF, W.C5_CV_HALF, W.C5_CV_HALF)[0] / q
-        if rounds >= 2 and abs(cd - prev) < tol:
+        prev, cd = cd, L.cv_force(W.C5_CV_HALF, W.C5_CV_HALF, W.C5_CV_HALF)[0] / q
+        if (rounds >= 2 and abs(cd - prev) < tol) or steps >= max_steps:
             break
-        if steps >= max_steps:
-            break
-        prev = cd
     L.close()
     return cd, steps
 
@@ -349,54 +382,79 @@ def host_driver_leg(A, cfg, frames=40, steps=10, fields=("rho", "ux", "uy", "uz"
     return res
 
 
-def served_leg(A, W, n=256, base_k=32, round_steps=200, tol=1e-4, max_steps=4000, seed=5150, load=4.0):
-    """256 seeded requests (32^3 .. 160x80x80, spheres / ellipsoids, 30 % thermal), FP16
-    storage.  One-at-a-time: the first base_k requests solved back to back on the plain
-    solver.  Served: all n arrive as a Poisson stream at `load` times the one-at-a-time
-    rate and run on the server; latency = completion - arrival."""
+def served_leg(A, W, precision="fp32", n=256, base_k=24, round_steps=200, tick=20, tol=1e-4, max_steps=20000,
+               seed=5150, load=0.7, slots=256):
+    """The C5 stream: 256 seeded requests (32^3 .. 160x80x80, spheres / ellipsoids of
+    6-20 cells, Re drawn from [50, 500], 30 % thermal with Ra in [1800, 5000], free-slip
+    side walls), each retired at |dCd| < tol over a 200-step round or at max_steps.
+    One-at-a-time: the first base_k requests solved back to back on the plain solver.
+    Saturation: all n submitted at once; solves/s = n / makespan.  Latency: the n
+    requests arrive as a Poisson stream at `load` times the saturation rate, and
+    latency = completion - arrival (time to converged Cd, queueing included)."""
     reqs = W.c5_requests(n, seed)
+    cfgs = [A.default_config(**cfg, precision=precision) for cfg, _ in reqs]
     t0 = time.perf_counter()
-    alone = [solve_alone(A, W, cfg, round_steps, tol, max_steps) for cfg, _ in reqs[:base_k]]
+    alone = [solve_alone(A, W, cfg, precision, round_steps, tol, max_steps) for cfg, _ in reqs[:base_k]]
     t_alone = time.perf_counter() - t0
     base_rate = base_k / t_alone
-    log(f"[served] one-at-a-time: {base_k} requests in {t_alone:.2f} s = {base_rate:.3f} solves/s")
-    lam = load * base_rate
-    rng = np.random.Generator(np.random.PCG64(seed + 1))
-    gaps = rng.exponential(1.0 / lam, size=n)
-    srv = A.Server(precision="fp16", round_steps=round_steps, cd_tolerance=tol, max_steps=max_steps,
-                   cv_half=W.C5_CV_HALF)
-    cfgs = [A.default_config(**cfg, precision="fp16") for cfg, _ in reqs]
-    ids = []
-    start = srv.clock()
-    t_next = start
-    for k in range(n):
-        t_next += gaps[k]
-        while srv.clock() < t_next:
-            time.sleep(min(0.002, max(0.0, t_next - srv.clock())))
-        ids.append(srv.submit(cfgs[k]))
-    if not srv.wait_all(600.0):
-        raise RuntimeError("served requests did not finish")
-    res = [srv.poll(i) for i in ids]
-    st = srv.stats()
-    srv.close()
-    lat = np.array([r["t_done"] - r["t_submit"] for r in res])
-    makespan = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+    log(f"[served {precision}] one-at-a-time: {base_k} requests in {t_alone:.2f} s = {base_rate:.3f} solves/s")
+
+    def run(gaps):
+        srv = A.Server(precision=precision, round_steps=round_steps, tick_steps=tick, cd_tolerance=tol,
+                       max_steps=max_steps, cv_half=W.C5_CV_HALF, max_cases=slots)
+        ids = []
+        t_next = srv.clock()
+        for k in range(n):
+            t_next += gaps[k]
+            while srv.clock() < t_next:
+                time.sleep(min(0.002, max(0.0, t_next - srv.clock())))
+            ids.append(srv.submit(cfgs[k]))
+        if not srv.wait_all(900.0):
+            raise RuntimeError("served requests did not finish")
+        res = [srv.poll(i) for i in ids]
+        st = srv.stats()
+        srv.close()
+        makespan = max(r["t_done"] for r in res) - min(r["t_submit"] for r in res)
+        lat = np.array([r["t_done"] - r["t_submit"] for r in res])
+        return res, st, makespan, lat
+
+    res_s, st_s, span_s, _ = run(np.zeros(n))
+    sat_rate = n / span_s
+    log(f"[served {precision}] saturation: {n} requests in {span_s:.2f} s = {sat_rate:.3f} solves/s, "
+        f"peak resident {st_s['peak_resident']}")
+    lam = load * sat_rate
+    gaps = np.random.Generator(np.random.PCG64(seed + 1)).exponential(1.0 / lam, size=n)
+    res, st, span, lat = run(gaps)
+    wait = np.array([r["t_admit"] - r["t_submit"] for r in res])
     same = sum(1 for k in range(base_k) if np.float64(res[k]["cd"]).tobytes() == np.float64(alone[k][0]).tobytes()
                and res[k]["steps"] == alone[k][1])
-    served_rate = n / makespan
-    log(f"[served] {n} requests at lambda {lam:.2f}/s: {served_rate:.3f} solves/s, p50 {np.percentile(lat, 50):.2f} s, "
-        f"p99 {np.percentile(lat, 99):.2f} s, peak resident {st['peak_resident']}")
-    return {"requests": n, "lambda_per_s": round(lam, 3), "solves_per_s": round(served_rate, 3),
-            "p50_s": round(float(np.percentile(lat, 50)), 3), "p99_s": round(float(np.percentile(lat, 99)), 3),
-            "one_at_a_time_solves_per_s": round(base_rate, 3), "one_at_a_time_sample": base_k,
-            "speedup_vs_one_at_a_time": round(served_rate / base_rate, 2),
-            "converged": sum(r["state_name"] == "done" for r in res),
-            "stopped_at_cap": sum(r["state_name"] == "limit" for r in res),
-            "failed": sum(r["state_name"] == "failed" for r in res),
-            "identical_to_alone": f"{same}/{base_k}",
-            "served_mlups": round(st["cell_steps"] / max(st["busy_seconds"], 1e-9) / 1e6, 1),
-            "peak_resident": st["peak_resident"], "peak_bytes": st["peak_bytes"],
-            "rule": f"retire at |dCd| < {tol} over a {round_steps}-step round or at {max_steps} steps; FP16 storage"}
+    done = [r for r in res if r["state_name"] == "done"]
+    steps_done = np.array([r["steps"] for r in done]) if done else np.zeros(1)
+    re_req = np.array([m["re"] for _, m in reqs])
+    re_eff = np.array([m["re_eff"] for _, m in reqs])
+    out = {"precision": precision, "requests": n,
+           "saturation_solves_per_s": round(sat_rate, 3), "saturation_makespan_s": round(span_s, 2),
+           "one_at_a_time_solves_per_s": round(base_rate, 3), "one_at_a_time_sample": base_k,
+           "speedup_vs_one_at_a_time": round(sat_rate / base_rate, 2),
+           "poisson_lambda_per_s": round(lam, 3), "poisson_load": load,
+           "p50_s": round(float(np.percentile(lat, 50)), 3), "p99_s": round(float(np.percentile(lat, 99)), 3),
+           "p50_wait_for_admission_s": round(float(np.percentile(wait, 50)), 4),
+           "converged": len(done), "stopped_at_cap": sum(r["state_name"] == "limit" for r in res),
+           "failed": sum(r["state_name"] == "failed" for r in res),
+           "steps_to_converge_p50": int(np.percentile(steps_done, 50)),
+           "steps_to_converge_max": int(steps_done.max()),
+           "converged_at_saturation": sum(r["state_name"] == "done" for r in res_s),
+           "identical_to_alone": f"{same}/{base_k}",
+           "served_mlups": round(st["cell_steps"] / max(st["busy_seconds"], 1e-9) / 1e6, 1),
+           "peak_resident": st["peak_resident"], "peak_resident_at_saturation": st_s["peak_resident"],
+           "peak_bytes": st["peak_bytes"], "slots": slots,
+           "re_requested": [round(float(re_req.min()), 1), round(float(re_req.max()), 1)],
+           "re_realised": [round(float(re_eff.min()), 1), round(float(re_eff.max()), 1)],
+           "re_realised_equals_requested": int(np.sum(np.abs(re_eff - re_req) < 1e-6 * re_req)),
+           "rule": f"retire at |dCd| < {tol} over a {round_steps}-step round or at {max_steps} steps; "
+                   f"{tick}-step ticks (requests join between ticks); tau = 1/2 + 3 U0 D / Re, at least 0.51"}
+    log(f"[served {precision}] {out}")
+    return out
 
 
 # ------------------------------------------------------------------ HBM traffic
@@ -482,7 +540,10 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 counter passes")
     ap.add_argument("--no-served", action="store_true", help="skip the served-mode leg")
     ap.add_argument("--no-studies", action="store_true",
-                    help="skip the host-driver leg and the precision and Nusselt studies of the secondary section")
+                    help="skip the host-driver leg and the short precision studies of the secondary section")
+    ap.add_argument("--studies", action="store_true",
+                    help="add the long studies: 33.5M cells for 20k steps in five storages, and the Rayleigh-Benard "
+                         "Nusselt number after 300k steps in four")
     ap.add_argument("--pmc-child", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-case", default=HEADLINE_CASE, choices=sorted(PMC_CASES), help=argparse.SUPPRESS)
     args = ap.parse_args()
@@ -558,11 +619,12 @@ def main():
         sec["c2_fp32"] = line(run_gpu(A, c2, 1000, 50, "c2 fp32"), BYTES_PER_CELL[("fp32", True)])
         sec["c2_fp32"]["tau"] = round(c2["tau"], 5)
         sec["c4_fp32"] = line(r, bpc)
-        for prec in ("fp16", "bf16"):
+        for prec in ("fp16", "bf16", "fp16_eq", "bf16_eq", "fp32_fp64c"):
             c4p = dict(c4, precision=prec)
-            sec[f"c4_{prec}"] = line(run_gpu(A, c4p, 200, 20, f"c4 {prec}"), BYTES_PER_CELL[(prec, True)])
-        sec["fp16_over_fp32"] = round(sec["c4_fp16"]["mlups"] / r["mlups"], 3)
-        sec["bf16_over_fp32"] = round(sec["c4_bf16"]["mlups"] / r["mlups"], 3)
+            bpc4 = BYTES_PER_CELL[("fp32", True)] if prec == "fp32_fp64c" else BYTES_PER_CELL[("fp16", True)]
+            sec[f"c4_{prec}"] = line(run_gpu(A, c4p, 200, 20, f"c4 {prec}"), bpc4)
+        for prec in ("fp16", "bf16", "fp16_eq", "bf16_eq", "fp32_fp64c"):
+            sec[f"{prec}_over_fp32"] = round(sec[f"c4_{prec}"]["mlups"] / r["mlups"], 3)
         sec["c1_fp64"] = line(run_gpu(A, W.c1_sphere_fp64(), 1000, 50, "c1 fp64"), BYTES_PER_CELL[("fp64", False)])
         sec["c1_fp64"].update(c1_results(A, W))
         c3cfg, c3T = W.c3_rayleigh_benard()
@@ -575,13 +637,17 @@ def main():
                             "c3": "128x64x128 coupled D3Q19+D3Q7 FP32, Rayleigh-Benard Ra 2500 Pr 0.71"}
         if not args.no_studies:
             sec["host_driver"] = host_driver_leg(A, c2)
-            sec["c4_precision_study"] = precision_study(A, W)
             sec["c2_re100_precision_study"] = precision_study(A, W, scale=1.0, re=100.0, steps=16000, window=4000,
                                                               every=250)
+            sec["c1_precision_study"] = c1_precision(A, W)
+        if args.studies:   # about four minutes more
+            sec["c4_precision_study"] = precision_study(A, W)
             sec["c3_nusselt"] = c3_nusselt(A, W)
         out["secondary"] = sec
     if not args.no_served:
-        out["served"] = served_leg(A, W)
+        out["served"] = served_leg(A, W, "fp32")
+        out["served_fp16_eq"] = served_leg(A, W, "fp16_eq")
+        out["served_fp16"] = served_leg(A, W, "fp16", n=64, max_steps=6000, base_k=8)
     if not args.no_traffic:
         tr, err = measure_traffic()
         if tr:

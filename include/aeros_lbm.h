@@ -49,6 +49,10 @@ extern "C" {
 #define AEROS_FP32_FP64C 4   /* float deviations f_i - w_i and velocity stencil, double
                                 collision and thermal lattice: FP32's memory traffic, FP64's
                                 arithmetic (no counterpart in the reference)               */
+#define AEROS_FP16_EQ 5   /* IEEE half of 2^12 (f_i - e_i), e_i the inflow equilibrium
+                             f_i^eq(1, u_in): the free stream stores near zero, and the
+                             scale keeps small deviations out of the subnormal range     */
+#define AEROS_BF16_EQ 6   /* bfloat16 of f_i - e_i, e_i as for AEROS_FP16_EQ              */
 
 /* field identifiers for aeros_lbm_read_field */
 #define AEROS_FIELD_RHO        0   /* lbmRho        (aeros/src/lbm.h:132)          */
@@ -138,7 +142,9 @@ typedef struct aeros_lbm_config {
     int32_t free_slip;            /* faces closed by a free-slip (specular) wall, a mask of
                                      AEROS_FACE_*; none may lie on a periodic axis.  No
                                      counterpart in the reference, whose non-periodic faces
-                                     copy the cell's own population in place.              */
+                                     copy the cell's own population in place.  Acts on the
+                                     D3Q19 lattice; the D3Q7 thermal lattice keeps its own
+                                     shell rules.                                          */
 } aeros_lbm_config;
 
 /* Domain faces (aeros_lbm_config.free_slip). */

@@ -10,14 +10,16 @@
  * its arithmetic does not depend on the other cases in the launch.
  *
  * A server thread owns the device work.  Requests are queued by
- * aeros_serve_submit from any thread; the server admits them while the
- * resident cases' device memory stays within the budget and a slot is free,
- * advances all resident cases in rounds of round_steps steps (one launch per
- * step and (LES, thermal) group), evaluates each case's drag coefficient
- * after every round and retires a case when |Cd - Cd of the previous round|
- * < cd_tolerance (or at max_steps).  Admission, rounds and retirement
- * interleave, so requests that arrive while others run join at the next
- * round.
+ * aeros_serve_submit from any thread; builder threads admit them while the
+ * resident cases' device memory stays within the budget and a slot is free.
+ * The server advances all resident cases in ticks of tick_steps steps (one
+ * launch per step and (LES, thermal, free-slip) group, then one status copy
+ * for all cases).  Each case counts its rounds of round_steps steps from its
+ * own admission: its drag coefficient is evaluated at the end of each of its
+ * rounds, and it is retired when |Cd - Cd of its previous round| <
+ * cd_tolerance (or at max_steps).  A request that arrives while others run
+ * joins at the next tick, and a retired case's slot and memory are free for
+ * the next request at once.
  */
 #ifndef AEROS_SERVE_H
 #define AEROS_SERVE_H
@@ -34,7 +36,8 @@ typedef struct aeros_server aeros_server;
 
 typedef struct aeros_serve_config {
     int32_t struct_size;          /* = sizeof(aeros_serve_config)                          */
-    int32_t precision;            /* storage of every case: AEROS_FP16 (default) or AEROS_FP32 */
+    int32_t precision;            /* storage of every case: AEROS_FP16 (default), AEROS_FP16_EQ
+                                     or AEROS_FP32                                           */
     int64_t memory_budget;        /* device bytes the resident cases may use (0 = 3/4 of free) */
     int32_t max_cases;            /* resident cases at most (slots)                        */
     int32_t round_steps;          /* steps between drag evaluations                        */
@@ -42,6 +45,11 @@ typedef struct aeros_serve_config {
     int32_t max_steps;            /* retire unconverged cases after this many steps        */
     float   cv_half;              /* control-volume half extent, fraction of the domain    */
     int32_t device;               /* HIP device ordinal                                    */
+    int32_t tick_steps;           /* steps per batched launch sequence; requests join, and
+                                     the drag is sampled, between ticks; divides
+                                     round_steps (0 = round_steps)                        */
+    int32_t cd_average;           /* 0: a round's Cd is the drag after its last step;
+                                     1: the mean of the drag sampled after each tick        */
 } aeros_serve_config;
 
 /* request states */
@@ -64,19 +72,23 @@ typedef struct aeros_serve_result {
     float   nusselt;              /* thermal cases                                       */
     double  t_submit, t_admit, t_done;   /* seconds on the server's steady clock        */
     int64_t bytes;                /* device memory the case held                         */
+    int64_t tick_admit, tick_done;   /* the server's tick count when the case joined and
+                                        when it was retired                               */
 } aeros_serve_result;
 
 typedef struct aeros_serve_stats {
     int64_t submitted, completed, failed;
-    int64_t rounds, launches;
+    int64_t rounds, launches;     /* rounds of the retired cases; step launches           */
+    int64_t ticks;
     int32_t resident, queued, peak_resident;
     int64_t bytes_in_use, peak_bytes;
-    double  busy_seconds;         /* wall time spent in rounds                           */
+    double  busy_seconds;         /* wall time spent in ticks                            */
     int64_t cell_steps;           /* sum over cases of cells x steps                     */
 } aeros_serve_stats;
 
 /* Defaults: FP16, 3/4 of free device memory, 64 slots, 200-step rounds,
- * tolerance 1e-4, 200000 steps, control volume 0.30. */
+ * tolerance 1e-4, 200000 steps, control volume 0.30, 20-step ticks, Cd at
+ * the end of each round. */
 void aeros_serve_config_defaults(aeros_serve_config* cfg);
 
 /* Starts the server thread.  NULL on failure with *err set. */

@@ -338,18 +338,27 @@ public:
     int yLo = 1, yHi = 0; float H = 1, tauT = 0.7f;
     float tNu = 0, tAlpha = 0, tRa = 0, tPr = 0, tG = 0, tNusselt = 0, tMeanT = 0.5f;
 
+    // inflow equilibrium f_q^eq(1, u_in): the shift of the equilibrium-shifted
+    // 16-bit storages (no counterpart in the reference)
+    R eqs[Q] = {};
     bool quantised() const {
-        return storage == AEROS_FP16 || storage == AEROS_BF16 || storage == AEROS_FP32_FP64C;
+        return storage == AEROS_FP16 || storage == AEROS_BF16 || storage == AEROS_FP32_FP64C ||
+               storage == AEROS_FP16_EQ || storage == AEROS_BF16_EQ;
     }
     R qz(R x, int q) const {   // storage quantisation of a population
         if (storage == AEROS_FP32_FP64C) return K<R>::w(q) + (R)(float)(x - K<R>::w(q));
         if (storage == AEROS_FP16) return K<R>::w(q) + (R)q_half((float)(x - K<R>::w(q)));
         if (storage == AEROS_BF16) return K<R>::w(q) + (R)q_bf16((float)(x - K<R>::w(q)));
+        // stored half(float(f - e) * 2^12), read h * 2^-12 + e (one float rounding each)
+        if (storage == AEROS_FP16_EQ)
+            return eqs[q] + (R)(q_half((float)(x - eqs[q]) * 4096.0f) * (1.0f / 4096.0f));
+        if (storage == AEROS_BF16_EQ) return eqs[q] + (R)q_bf16((float)(x - eqs[q]));
         return x;
     }
     R qu(R x) const {          // storage quantisation of the velocity stencil input
         if (storage == AEROS_FP32_FP64C) return (R)(float)x;
-        if (storage == AEROS_FP16 || storage == AEROS_BF16) return (R)q_half((float)x);
+        if (storage == AEROS_FP16 || storage == AEROS_BF16 || storage == AEROS_FP16_EQ || storage == AEROS_BF16_EQ)
+            return (R)q_half((float)x);
         return x;
     }
     void quantize_cell(std::vector<R>& a, int cell) {
@@ -549,6 +558,10 @@ public:
             for (int q = 0; q < Q; ++q) f[(size_t)cc * Q + q] = eq_guarded<R>(q, R(1), u0, u1, u2, us);
         }
         std::fill(fNext.begin(), fNext.end(), R(0));
+        {   // the inflow equilibrium exactly as the inlet pull forms it
+            const R u0 = inUx, u1 = inUy, u2 = inUz, us = u0 * u0 + u1 * u1 + u2 * u2;
+            for (int q = 0; q < Q; ++q) eqs[q] = eq_guarded<R>(q, R(1), u0, u1, u2, us);
+        }
         if (quantised())
             for (int cc = 0; cc < total; ++cc) { quantize_cell(f, cc); quantize_cell(fNext, cc); }
 
@@ -1355,6 +1368,15 @@ float ora_last_divergence(void* p, int* step) {
 
 float ora_sutherland(float t_kelvin) { return sutherland(t_kelvin); }
 float ora_q_half(float x) { return q_half(x); }
+// The values the stored populations are taken relative to (w_q, or the inflow
+// equilibrium for the equilibrium-shifted storages).
+int ora_storage_shift(void* p, double* out) {
+    Handle* h = (Handle*)p;
+    const bool eq = h->prec == AEROS_FP16_EQ || h->prec == AEROS_BF16_EQ;
+    for (int q = 0; q < Q; ++q)
+        out[q] = eq ? (double)h->s32->eqs[q] : (h->s64 ? (double)K<double>::w(q) : (double)K<float>::w(q));
+    return 0;
+}
 float ora_q_bf16(float x) { return q_bf16(x); }
 
 // Voxelize on an arbitrary grid (aeros/src/voxel_grid.cpp:313-351 restated):

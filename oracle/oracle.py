@@ -55,6 +55,7 @@ def load() -> C.CDLL:
         "ora_thermal_reset": (C.c_int, [vp]),
         "ora_equilibrium_moments": (C.c_int, [C.c_float] * 4 + [P(C.c_float)] * 4),
         "ora_q_half": (C.c_float, [C.c_float]),
+        "ora_storage_shift": (C.c_int, [vp, P(C.c_double)]),
         "ora_sutherland": (C.c_float, [C.c_float]),
         "ora_q_bf16": (C.c_float, [C.c_float]),
         "ora_voxelize": (C.c_int, [P(C.c_float), C.c_int64, C.c_int32, C.c_int32, C.c_int32,
@@ -151,6 +152,13 @@ class OracleLattice:
         s = A.Scalars()
         self.lib.ora_scalars(self.h, C.byref(s))
         return s.as_dict()
+
+    def storage_shift(self):
+        """The 19 values stored populations are taken relative to (w_q, or the inflow
+        equilibrium for the equilibrium-shifted 16-bit storages)."""
+        out = np.empty(19, dtype=np.float64)
+        self.lib.ora_storage_shift(self.h, out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out
 
     def cv_force(self, hx=0.30, hy=0.30, hz=0.30):
         out = (C.c_double * 3)()

@@ -92,7 +92,6 @@ def test_uniform_stream_passes_free_slip_walls_unchanged(oracle):
     (dict(free_slip=64), "unknown face"),
     (dict(free_slip=A.FACE_X0, periodic_x=1), "face of a periodic axis"),
     (dict(free_slip=A.FACE_Z1, periodic_z=1), "face of a periodic axis"),
-    (dict(free_slip=A.FACE_Y0, thermal=1), "thermal"),
 ])
 def test_invalid_free_slip_is_rejected_before_the_device(over, why):
     with pytest.raises(A.AerosError) as e:

@@ -74,11 +74,18 @@ def test_free_slip_16bit_runs_the_slip_kernel(oracle, prec):
         assert err < (2e-3 if prec == "fp16" else 8e-3), (name, err)
 
 
-def test_served_mode_rejects_free_slip():
-    srv = A.Server(precision="fp16")
-    try:
-        with pytest.raises(A.AerosError):
-            srv.submit(A.default_config(**W.slip_box(32, 32, 32, (10.0, 16.0, 16.0), 6.0, W.SIDE_WALLS),
-                                        precision="fp16"))
-    finally:
-        srv.close()
+@pytest.mark.parametrize("les", [0, 1])
+def test_free_slip_with_the_thermal_lattice_fp32_bitwise(oracle, les):
+    """Free-slip side walls on the D3Q19 lattice with the D3Q7 lattice beside it
+    (hot and cold y walls, x and z wrapping for the temperature), as the served
+    thermal requests run: fields, temperature and populations bit for bit."""
+    rng = np.random.Generator(np.random.PCG64(21))
+    cfg, _ = W.c5_request(rng, thermal_frac=1.0, size=0.0)
+    cfg["use_les"] = les
+    G, O = make_pair(oracle, **cfg)
+    for n in (1, 39):
+        G.step(n)
+        O.step(n)
+        assert_same(G, O, FIELDS + ("T", "buoyancy"))
+    np.testing.assert_array_equal(G.populations(), O.populations())
+    assert G.scalars()["nusselt"] == O.scalars()["nusselt"]

@@ -17,7 +17,8 @@ import workloads as W
 pytestmark = pytest.mark.gpu
 
 # precision: (|Cd error| bound, velocity L2 bound), both relative to FP64
-TOL = {"fp32": (0.01, 5e-4), "fp16": (0.15, 1e-2), "bf16": (0.12, 3e-2)}
+TOL = {"fp32": (0.01, 5e-4), "fp16": (0.15, 1e-2), "bf16": (0.12, 3e-2), "fp32_fp64c": (1e-4, 1e-6),
+       "fp16_eq": (0.03, 1.5e-3), "bf16_eq": (0.06, 6e-3)}
 
 
 def _run(prec):
