@@ -235,14 +235,32 @@ def line(r, bpc):
 
 
 def cpu_model():
-    """The host CPU's model name and the threads this process may use."""
+    """The host CPU's model name (with its family / model / stepping numbers, which
+    name the generation where a virtual machine shortens the name) and the threads
+    this process may use."""
     model = "unknown"
+    info = {}
     try:
         for ln in open("/proc/cpuinfo"):
-            if ln.startswith("model name"):
-                model = ln.split(":", 1)[1].strip()
+            k, _, v = ln.partition(":")
+            k = k.strip()
+            if k in ("model name", "cpu family", "model", "stepping") and k not in info:
+                info[k] = v.strip()
+            if not ln.strip() and info:
                 break
     except OSError:
+        pass
+    if info:
+        model = (f"{info.get('model name', 'unknown')} (family {info.get('cpu family', '?')}, "
+                 f"model {info.get('model', '?')}, stepping {info.get('stepping', '?')})")
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:") or ln.startswith("BIOS Model name:"):
+                v = ln.split(":", 1)[1].strip()
+                if v and v not in model:
+                    model += f"; lscpu {ln.split(':', 1)[0]}: {v}"
+    except (OSError, subprocess.SubprocessError):
         pass
     try:
         avail = len(os.sched_getaffinity(0))
