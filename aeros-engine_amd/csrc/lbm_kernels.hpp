@@ -176,18 +176,18 @@ template <class S, class R> __device__ __forceinline__ R sshift(const StepParams
 // The byte offset of element idx, hidden from the optimiser: the element
 // address is then the plane's uniform byte offset plus this one per-cell value
 // (one vector add per access), where it would otherwise form (plane + idx) *
-// size with an add and a shift per access.  The uniform plane offsets cost
-// scalar registers, so which form is faster depends on what else the kernel
-// holds; measured at 33.5M cells (TRT + LES): FP32 1.177 against 1.185 ms,
-// the equilibrium-shifted half storage 0.814 against 0.893 ms (90 VGPRs, 5
-// waves per SIMD, against 4), the plain half storage 0.891 against 0.800 ms
-// (97 VGPRs and 4 waves against 89 and 5).  Fields are the same bits either way.
+// size with an add and a shift per access.  Fields are the same bits either
+// way; measured at 33.5M cells (TRT + LES): FP32 1.185 -> 1.171 ms, and with
+// 5 waves per SIMD asked for the 16-bit kernels (kStepWaves below; the
+// uniform plane offsets cost them registers, and without the request they
+// fall to 4 waves) FP16 0.804 -> 0.764 ms, BF16 0.811 -> 0.787 ms, FP16
+// shifted by the inflow equilibrium 0.893 -> 0.793 ms.
 __device__ __forceinline__ uint32_t cell_bytes(uint32_t idx, uint32_t size) {
     uint32_t b = idx * size;
     asm("" : "+v"(b));
     return b;
 }
-template <class S> constexpr bool kByteAddr = !std::is_same<S, _Float16>::value && !std::is_same<S, bf16_t>::value;
+template <class S> constexpr bool kByteAddr = true;
 template <class S> struct Lat {
     S* p;
     __device__ Lat(const void* base) : p(static_cast<S*>(const_cast<void*>(base))) {}
@@ -1086,8 +1086,17 @@ __global__ __launch_bounds__(BLOCK) void k_fold(StepParams<R> P, StepBuffers B, 
 
 // SLIP: some domain face carries a free-slip wall (a separate instance, so
 // that the reflection's addressing costs the other lattices no registers).
+// Waves per SIMD asked of the compiler: at least 5 for the 16-bit storages
+// without the thermal lattice (their natural allocation lands a few registers
+// above 96, i.e. 4 waves; asked, the plain LES kernels fit 80-90 and run 6 or
+// 5), except where the request makes the compiler spill (a body force; free
+// slip without LES); no request otherwise.
+template <class S, bool LES, bool THERM, bool FORCE, bool SLIP>
+constexpr int kStepWaves = (sizeof(S) == 2 && !THERM && !FORCE && (!SLIP || LES)) ? 5 : 1;
+
 template <class R, class S, class U, bool LES, bool THERM, bool FORCE, bool SLIP = false>
-__global__ __launch_bounds__(BLOCK) void k_step(StepParams<R> P, StepBuffers B) {
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(kStepWaves<S, LES, THERM, FORCE, SLIP>)))
+void k_step(StepParams<R> P, StepBuffers B) {
     __shared__ RedLds L;
     step_grid<R, S, U, LES, THERM, FORCE, SLIP>(P, B, StepFlags{P.first, P.write_fields, P.dry, P.k, P.pending},
                                                 blockIdx.x, gridDim.x, L);
@@ -1111,7 +1120,8 @@ struct BlockMap {
 };
 
 template <class R, class S, class U, bool LES, bool THERM, bool FORCE, bool SLIP>
-__global__ __launch_bounds__(BLOCK) void k_step_batch(const BatchCase<R>* __restrict__ cases,
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(kStepWaves<S, LES, THERM, FORCE, SLIP>)))
+void k_step_batch(const BatchCase<R>* __restrict__ cases,
                                                       const BlockMap* __restrict__ map, int i, int nsteps) {
     __shared__ RedLds L;
     // block-uniform, read-only: scalar loads, so the case's parameters sit in
