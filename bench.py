@@ -487,7 +487,8 @@ def device_copy(nbytes, reps):
 
 
 PMC_CASES = {"c2": dict(scale=1.0, precision="fp32"), "c4_fp32": dict(scale=2.0, precision="fp32"),
-             "c4_fp16": dict(scale=2.0, precision="fp16"), "c4_bf16": dict(scale=2.0, precision="bf16")}
+             "c4_fp16": dict(scale=2.0, precision="fp16"), "c4_bf16": dict(scale=2.0, precision="bf16"),
+             "c4_fp16_eq": dict(scale=2.0, precision="fp16_eq")}
 HEADLINE_CASE = "c4_fp32"
 
 

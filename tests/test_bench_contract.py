@@ -31,5 +31,5 @@ def test_line_reports_bandwidth_from_the_kernel_time():
 def test_counter_cases_name_the_measured_lattices():
     import bench
     assert bench.PMC_CASES["c2"] == dict(scale=1.0, precision="fp32")
-    assert {c["precision"] for c in bench.PMC_CASES.values()} == {"fp32", "fp16", "bf16"}
+    assert {c["precision"] for c in bench.PMC_CASES.values()} == {"fp32", "fp16", "bf16", "fp16_eq"}
     assert np.isclose(bench.HBM_PEAK_GBS, 8000.0)
