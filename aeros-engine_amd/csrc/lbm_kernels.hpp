@@ -562,7 +562,7 @@ struct StepFlags {
 // kernels: there the uniform plane offsets exhaust the scalar registers, the
 // compiler no longer unrolls the reflection's direction loop, and the step
 // of a 39x35x35 lattice took 37 us instead of 11 (FP32).
-template <class R, class S, bool LES, bool THERM, bool FORCE, bool SLIP>
+template <bool SLIP>
 constexpr bool kByteAddr = !SLIP;
 
 // One lattice step over blocks blk of nblk (the whole grid for a single
@@ -1110,7 +1110,7 @@ template <class R, class S, class U, bool LES, bool THERM, bool FORCE, bool SLIP
 __global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(kStepWaves<S, LES, THERM, FORCE, SLIP>)))
 void k_step(StepParams<R> P, StepBuffers B) {
     __shared__ RedLds L;
-    step_grid<R, S, U, LES, THERM, FORCE, SLIP, kByteAddr<R, S, LES, THERM, FORCE, SLIP>>(
+    step_grid<R, S, U, LES, THERM, FORCE, SLIP, kByteAddr<SLIP>>(
         P, B, StepFlags{P.first, P.write_fields, P.dry, P.k, P.pending}, blockIdx.x, gridDim.x, L);
 }
 
@@ -1144,8 +1144,7 @@ void k_step_batch(const BatchCase<R>* __restrict__ cases,
     const StepFlags F{(C.pre && i == 0) ? 1 : 0, (i == nsteps - 1) ? 1 : 0, 0, k, i > 0 ? 1 : 0};
     StepBuffers B = C.buf[k & 1];
     if (F.first) B.f_in = B.f_out;
-    step_grid<R, S, U, LES, THERM, FORCE, SLIP, kByteAddr<R, S, LES, THERM, FORCE, SLIP>>(C.P, B, F, bm.blk,
-                                                                                         C.nblocks, L);
+    step_grid<R, S, U, LES, THERM, FORCE, SLIP, kByteAddr<SLIP>>(C.P, B, F, bm.blk, C.nblocks, L);
 }
 // The end of a served tick: one block per case folds its last step and
 // copies the case's status to out[case], so that one copy brings every
