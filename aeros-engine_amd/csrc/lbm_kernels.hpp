@@ -608,6 +608,12 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
     uint32_t m_next = cell < cend ? at(B.mask, cell) : 0u;
     for (; cell < cend; cell += BLOCK) {
         const uint32_t m = m_next;
+        // The population planes' offsets q n: with BF16 storage recomputed per
+        // cell from a copy of n the optimiser cannot hoist, so that they do not
+        // hold 38 scalar registers across the loop (which spilled into vector
+        // lanes: 3.4 % faster at 33.5M cells; FP16 and FP32 were slower so).
+        uint32_t n_it = n;
+        if constexpr (std::is_same<S, bf16_t>::value) asm volatile("" : "+s"(n_it));
         if (cell + BLOCK < cend) m_next = at(B.mask, cell + BLOCK);
         if (m & M_SOLID) continue;   // solid/ground: fields are constant (rho 1, u 0), never stored
         const bool inlet = m & P.inlet_face, outlet = m & P.outlet_face;
@@ -661,18 +667,18 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
         R f[Q];
         if (F.first) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) f[q] = lin.ld(cell, (uint32_t)q * n, sshift<S>(P, q));
+            for (int q = 0; q < Q; ++q) f[q] = lin.ld(cell, (uint32_t)q * n_it, sshift<S>(P, q));
         } else if (!(m & M_FACES)) {
             // interior: the source of q is cell - c_q; a uniform shifted base per direction
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const int off = cx_of(q) + cy_of(q) * nx + cz_of(q) * (int)sz;
-                f[q] = lin.ld(cell, (uint32_t)q * n - (uint32_t)off, sshift<S>(P, q));
+                f[q] = lin.ld(cell, (uint32_t)q * n_it - (uint32_t)off, sshift<S>(P, q));
             }
             if (m & M_LINKS) {   // half-way bounce-back: the cell's own opposite population
 #pragma unroll
                 for (int q = 1; q < Q; ++q)
-                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, sshift<S>(P, opp_of(q)));
+                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n_it, sshift<S>(P, opp_of(q)));
             }
         } else if (m & M_PULL_INLET) {
 #pragma unroll
@@ -684,9 +690,9 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
                 const R w = sshift<S>(P, q);
-                const R fo = lin.ld(cell, (uint32_t)q * n, w);
-                const R fi = lin.ld(ci, (uint32_t)q * n, w);
-                const R fi2 = lin.ld(ci2, (uint32_t)q * n, w);
+                const R fo = lin.ld(cell, (uint32_t)q * n_it, w);
+                const R fi = lin.ld(ci, (uint32_t)q * n_it, w);
+                const R fi2 = lin.ld(ci2, (uint32_t)q * n_it, w);
                 f[q] = fo + P.conv_c * (fi - fo) + P.conv_k2 * (fi - fi2);
             }
         } else if (!(m & (SLIP ? (P.wrap_faces | P.slip_faces) : P.wrap_faces))) {
@@ -697,12 +703,12 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
             for (int q = 0; q < Q; ++q) {
                 const int off = cx_of(q) + cy_of(q) * nx + cz_of(q) * (int)sz;
                 const uint32_t src = (m & out_faces(q)) ? cell : cell - (uint32_t)off;
-                f[q] = lin.ld(src, (uint32_t)q * n, sshift<S>(P, q));
+                f[q] = lin.ld(src, (uint32_t)q * n_it, sshift<S>(P, q));
             }
             if (m & M_LINKS) {
 #pragma unroll
                 for (int q = 1; q < Q; ++q)
-                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, sshift<S>(P, opp_of(q)));
+                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n_it, sshift<S>(P, opp_of(q)));
             }
         } else {
             // domain shell on a periodic or free-slip face: a source beyond a
@@ -729,7 +735,7 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
                     const int dq = ((int)rx + (int)ry + (int)rz >= 2)
                                        ? opp_of(q)
                                        : (rx ? mirror_of(q, 1) : (ry ? mirror_of(q, 2) : (rz ? mirror_of(q, 4) : q)));
-                    f[q] = lin.ld(out ? cell : src, (uint32_t)(out ? q : dq) * n, sshift<S>(P, out ? q : dq));
+                    f[q] = lin.ld(out ? cell : src, (uint32_t)(out ? q : dq) * n_it, sshift<S>(P, out ? q : dq));
                 }
             } else {
 #pragma unroll
@@ -740,13 +746,13 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
                     const uint32_t wrap = ((m & xf) ? (uint32_t)(cx_of(q) * nx) : 0u) +
                                           ((m & zf) ? (uint32_t)(cz_of(q) * nz) * sz : 0u);
                     const uint32_t src = out ? cell : cell - (uint32_t)off + wrap;
-                    f[q] = lin.ld(src, (uint32_t)q * n, sshift<S>(P, q));
+                    f[q] = lin.ld(src, (uint32_t)q * n_it, sshift<S>(P, q));
                 }
             }
             if (m & M_LINKS) {   // the link bits refer to the wrapped or reflected source
 #pragma unroll
                 for (int q = 1; q < Q; ++q)
-                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n, sshift<S>(P, opp_of(q)));
+                    if ((m >> q) & 1u) f[q] = lin.ld(cell, (uint32_t)opp_of(q) * n_it, sshift<S>(P, opp_of(q)));
             }
         }
 
@@ -960,7 +966,7 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
         }
 
 #pragma unroll
-        for (int q = 0; q < Q; ++q) lout.st(cell, (uint32_t)q * n, f[q], sshift<S>(P, q));
+        for (int q = 0; q < Q; ++q) lout.st(cell, (uint32_t)q * n_it, f[q], sshift<S>(P, q));
 
         // ---------------- velocity and diagnostics (aeros/src/lbm.cpp:1266-1305)
         R vx = ux, vy = uy, vz = uz;   // the stored velocity includes F/(2 rho)
