@@ -608,12 +608,15 @@ __device__ __forceinline__ void step_grid(const StepParams<R>& P, const StepBuff
     uint32_t m_next = cell < cend ? at(B.mask, cell) : 0u;
     for (; cell < cend; cell += BLOCK) {
         const uint32_t m = m_next;
-        // The population planes' offsets q n: with BF16 storage recomputed per
-        // cell from a copy of n the optimiser cannot hoist, so that they do not
-        // hold 38 scalar registers across the loop (which spilled into vector
-        // lanes: 3.4 % faster at 33.5M cells; FP16 and FP32 were slower so).
+        // The population planes' offsets q n: for the two BF16 storages and
+        // the thermal kernels recomputed per cell from a copy of n the
+        // optimiser cannot hoist, so that they do not hold 38 scalar registers
+        // across the loop (which spilled into vector lanes).  At 33.5M cells
+        // BF16 3.4 % and shifted BF16 5.6 % faster, the thermal lattice about
+        // 1 %; FP16 and FP32 were slower so.
         uint32_t n_it = n;
-        if constexpr (std::is_same<S, bf16_t>::value) asm volatile("" : "+s"(n_it));
+        if constexpr (std::is_same<S, bf16_t>::value || std::is_same<S, b16e_t>::value || THERM)
+            asm volatile("" : "+s"(n_it));
         if (cell + BLOCK < cend) m_next = at(B.mask, cell + BLOCK);
         if (m & M_SOLID) continue;   // solid/ground: fields are constant (rho 1, u 0), never stored
         const bool inlet = m & P.inlet_face, outlet = m & P.outlet_face;
